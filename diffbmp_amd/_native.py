@@ -1,0 +1,253 @@
+"""ctypes binding of ``libdiffbmp_amd.so`` (C ABI in ``include/diffbmp_amd.h``).
+
+PyTorch is used for device memory and the current HIP stream only.  There is
+no fallback: if the library is missing or the device is not a GPU, every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdiffbmp_amd.so")
+
+DBMP_F32, DBMP_F16, DBMP_BF16 = 0, 1, 2
+_DT = {torch.float32: DBMP_F32, torch.float16: DBMP_F16, torch.bfloat16: DBMP_BF16}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class Prims(ctypes.Structure):
+    _fields_ = [("means2D", ctypes.c_void_p), ("radii", ctypes.c_void_p),
+                ("rotations", ctypes.c_void_p), ("opacities", ctypes.c_void_p),
+                ("colors", ctypes.c_void_p), ("bmp_sel", ctypes.c_void_p),
+                ("num_primitives", ctypes.c_int32), ("dtype", ctypes.c_int32)]
+
+
+class Bitmaps(ctypes.Structure):
+    _fields_ = [("templates", ctypes.c_void_p), ("colors_orig", ctypes.c_void_p),
+                ("num_templates", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("width", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("colors_orig_mode", ctypes.c_int32), ("c_blend", ctypes.c_float),
+                ("version", ctypes.c_int64)]
+
+
+class TileLists(ctypes.Structure):
+    _fields_ = [("tile_offsets", ctypes.c_void_p), ("tile_indices", ctypes.c_void_p)]
+
+
+EXPORTS = {
+    "dbmp_version": (ctypes.c_int, []),
+    "dbmp_last_error": (ctypes.c_char_p, []),
+    "dbmp_init": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                 ctypes.c_int]),
+    "dbmp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "dbmp_template_extents": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Bitmaps),
+                                             ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_bin": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims), ctypes.c_void_p,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                ctypes.c_void_p]),
+    "dbmp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims),
+                                    ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
+                                    ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims),
+                                     ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
+                                     ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dbmp_timing_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int64)]),
+    "dbmp_timing_reset": (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+
+class NativeError(RuntimeError):
+    """A libdiffbmp_amd call returned an error code."""
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built; run diffbmp_amd.build.build() "
+                    "(hipcc --offload-arch=gfx950). There is no CPU fallback.")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in EXPORTS.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().dbmp_last_error().decode(errors="replace")
+        raise NativeError(f"diffbmp_amd error {rc}: {msg}")
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(dev: torch.device):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def require_gpu(t: torch.Tensor, name: str) -> None:
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a GPU tensor (got device {t.device}); "
+                           "the rasterizer has no CPU path")
+
+
+class Handle:
+    """One native rasterizer (``dbmp_init``): canvas, tile size, alpha bound,
+    and device scratch.  Calls are ordered on the current torch stream."""
+
+    def __init__(self, image_height: int, image_width: int, tile_size: int = 16,
+                 sigma: float = 0.0, alpha_upper_bound: float = 1.0,
+                 max_prims_per_pixel: int = 151, num_primitives: int | None = None,
+                 device: torch.device | None = None):
+        L = lib()
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.H, self.W, self.ts = int(image_height), int(image_width), int(tile_size)
+        self.alpha_ub = float(alpha_upper_bound)
+        self.tilesX = -(-self.W // self.ts)
+        self.tilesY = -(-self.H // self.ts)
+        self.tiles = self.tilesX * self.tilesY
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(L.dbmp_init(ctypes.byref(h), self.H, self.W, self.ts, float(sigma), self.alpha_ub,
+                              int(max_prims_per_pixel), int(num_primitives or 0)))
+        self._h = h
+        self._L = L
+        self.pair_capacity = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._L.dbmp_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ------------------------------------------------------------------
+    @staticmethod
+    def prims(means2D, radii, rotations, opacities, colors, bmp_sel) -> Prims:
+        dt = _DT[means2D.dtype]
+        for t, nm in ((radii, "radii"), (rotations, "rotations"), (opacities, "opacities"),
+                      (colors, "colors")):
+            if t.dtype != means2D.dtype:
+                raise TypeError(f"{nm} dtype {t.dtype} != means2D dtype {means2D.dtype}")
+        N = radii.shape[0]
+        if means2D.shape != (N, 2) or colors.shape != (N, 3) or rotations.shape != (N,) \
+                or opacities.shape != (N,) or bmp_sel.shape != (N,):
+            raise ValueError("primitive tensors must be means2D (N,2), radii/rotations/opacities/"
+                             "bmp_sel (N,), colors (N,3)")
+        if bmp_sel.dtype != torch.int32:
+            raise TypeError("bmp_sel must be int32")
+        return Prims(means2D.data_ptr(), radii.data_ptr(), rotations.data_ptr(),
+                     opacities.data_ptr(), colors.data_ptr(), bmp_sel.data_ptr(), N, dt)
+
+    @staticmethod
+    def bitmaps(templates, colors_orig=None, colors_orig_mode=0, c_blend=0.0,
+                version: int = -1) -> Bitmaps:
+        if templates.dim() != 3:
+            raise ValueError("templates must be (T,Ht,Wt)")
+        T, Ht, Wt = templates.shape
+        co = 0
+        if colors_orig is not None and colors_orig_mode:
+            if colors_orig.dtype != templates.dtype:
+                raise TypeError("colors_orig must have the templates' dtype")
+            if colors_orig.shape[1:] != (Ht, Wt, 3):
+                raise ValueError("colors_orig must be (M,Ht,Wt,3)")
+            co = colors_orig.data_ptr()
+        return Bitmaps(templates.data_ptr(), co, T, Ht, Wt, _DT[templates.dtype],
+                       int(colors_orig_mode if co else 0), float(c_blend), int(version))
+
+    def template_extents(self, templates: torch.Tensor) -> torch.Tensor:
+        require_gpu(templates, "templates")
+        templates = templates.contiguous()
+        ext = torch.empty((templates.shape[0], 4), dtype=torch.float32, device=templates.device)
+        bm = self.bitmaps(templates)
+        check(self._L.dbmp_template_extents(self._h, ctypes.byref(bm), _ptr(ext),
+                                            _stream(templates.device)))
+        return ext
+
+    def bin_async(self, prims: Prims, ext: torch.Tensor, tmpl_shape, capacity: int,
+                  tile_offsets: torch.Tensor, tile_indices: torch.Tensor,
+                  pair_count: torch.Tensor) -> None:
+        """Launch binning into caller buffers; no host synchronisation."""
+        T, Ht, Wt = tmpl_shape
+        check(self._L.dbmp_bin(self._h, ctypes.byref(prims), _ptr(ext), int(T), int(Ht), int(Wt),
+                               _ptr(tile_offsets), _ptr(tile_indices), int(capacity),
+                               _ptr(pair_count), _stream(ext.device)))
+
+    def bin(self, prims: Prims, ext: torch.Tensor, tmpl_shape):
+        """Tile lists (offsets (tiles+1) int32, indices (P) int32).  Reads the
+        pair count back once (the only host synchronisation of the step) and
+        regrows the index buffer if it overflowed."""
+        dev = ext.device
+        N = prims.num_primitives
+        cap = max(self.pair_capacity, 4 * N + 1024)
+        offs = torch.empty(self.tiles + 1, dtype=torch.int32, device=dev)
+        cnt = torch.empty(1, dtype=torch.int64, device=dev)
+        for _ in range(2):
+            idx = torch.empty(cap, dtype=torch.int32, device=dev)
+            self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt)
+            P = int(cnt.item())
+            if P <= cap:
+                self.pair_capacity = max(self.pair_capacity, cap)
+                return offs, idx[:P]
+            cap = int(math.ceil(P * 1.25)) + 1024
+            if cap >= 2 ** 31:
+                raise RuntimeError(f"{P} (tile, primitive) pairs exceed the int32 index range")
+        raise RuntimeError("binning did not converge")  # unreachable
+
+    def forward(self, prims: Prims, bm: Bitmaps, offsets: torch.Tensor, indices: torch.Tensor,
+                eps_T: float, out_color, out_alpha, out_T, out_last) -> None:
+        lists = TileLists(offsets.data_ptr(), indices.data_ptr() if indices.numel() else offsets.data_ptr())
+        check(self._L.dbmp_forward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(lists),
+                                   float(eps_T), _ptr(out_color), _ptr(out_alpha), _ptr(out_T),
+                                   _ptr(out_last), _stream(out_color.device)))
+
+    def backward(self, prims: Prims, bm: Bitmaps, offsets, indices, eps_T, grad_color, grad_alpha,
+                 out_color, out_T, out_last, g_means, g_r, g_th, g_v, g_c) -> None:
+        lists = TileLists(offsets.data_ptr(), indices.data_ptr() if indices.numel() else offsets.data_ptr())
+        check(self._L.dbmp_backward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(lists),
+                                    float(eps_T), _ptr(grad_color), _ptr(grad_alpha), _ptr(out_color),
+                                    _ptr(out_T), _ptr(out_last), _ptr(g_means), _ptr(g_r), _ptr(g_th),
+                                    _ptr(g_v), _ptr(g_c), _stream(grad_color.device)))
+
+    # timing -------------------------------------------------------------
+    def timing_enable(self, on: bool = True) -> None:
+        check(self._L.dbmp_timing_enable(self._h, int(on)))
+
+    def timing_stats(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        fn, bn = ctypes.c_int64(), ctypes.c_int64()
+        check(self._L.dbmp_timing_stats(self._h, ctypes.byref(f), ctypes.byref(fn), ctypes.byref(b),
+                                        ctypes.byref(bn)))
+        return f.value, fn.value, b.value, bn.value
+
+    def timing_reset(self) -> None:
+        check(self._L.dbmp_timing_reset(self._h))

@@ -1,0 +1,373 @@
+// Binning: primitive -> 16x16-tile lists, ascending primitive id per tile.
+//
+// The reference computes the assignment with dense torch ops
+// (pydiffbmp/core/renderer/vector_renderer.py:194-252, 401-444) and packs it
+// with a Python loop over tiles (simple_tile_renderer.py:721-766).  Here:
+//   k_prep        one thread per primitive: rotated template-extent AABB in
+//                 exact fp32 (every torch op rounded separately), closed-interval
+//                 tile range, pair count, and the raster constants (PrimRec).
+//   k_scan_counts exclusive 64-bit scan of the pair counts (one workgroup).
+//   k_emit        one wave per primitive writes its (tile id, primitive id)
+//                 pairs, tiles in row-major order, primitives in id order.
+//   k_radix_*     stable LSD radix sort on the tile id; stability keeps the
+//                 primitive ids ascending inside each tile.
+//   k_tile_ranges CSR offsets by binary search over the sorted tile ids.
+#pragma once
+#include "dbmp_device.h"
+
+namespace dbmp {
+
+constexpr int kRadixThreads = 256;          // 4 waves
+constexpr int kRadixItems = 16;             // keys per thread
+constexpr int kRadixTile = kRadixThreads * kRadixItems;  // keys per block
+
+// Template extents (vector_renderer.py:151-192): min/max column/row of texels
+// > 1e-6 mapped through u = (col/(W-1))*2 - 1 in fp32; empty => (-1,1,-1,1).
+template <typename TT>
+__global__ __launch_bounds__(256) void k_template_extents(const TT* __restrict__ tmpl, int Ht,
+                                                          int Wt, float* __restrict__ ext) {
+  const int t = blockIdx.x;
+  const TT* m = tmpl + (long)t * Ht * Wt;
+  int cmin = Wt, cmax = -1, rmin = Ht, rmax = -1;
+  for (int i = threadIdx.x; i < Ht * Wt; i += blockDim.x) {
+    if (ldf(m, i) > 1e-6f) {
+      const int yy = i / Wt, xx = i - yy * Wt;
+      cmin = min(cmin, xx); cmax = max(cmax, xx);
+      rmin = min(rmin, yy); rmax = max(rmax, yy);
+    }
+  }
+  __shared__ int red[4];
+  if (threadIdx.x == 0) { red[0] = Wt; red[1] = -1; red[2] = Ht; red[3] = -1; }
+  __syncthreads();
+  atomicMin(&red[0], cmin); atomicMax(&red[1], cmax);
+  atomicMin(&red[2], rmin); atomicMax(&red[3], rmax);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float* e = ext + 4 * t;
+    if (red[1] < 0) {
+      e[0] = -1.f; e[1] = 1.f; e[2] = -1.f; e[3] = 1.f;
+    } else {
+      e[0] = ((float)red[0] / (float)(Wt - 1)) * 2.f - 1.f;
+      e[1] = ((float)red[1] / (float)(Wt - 1)) * 2.f - 1.f;
+      e[2] = ((float)red[2] / (float)(Ht - 1)) * 2.f - 1.f;
+      e[3] = ((float)red[3] / (float)(Ht - 1)) * 2.f - 1.f;
+    }
+  }
+}
+
+__device__ __forceinline__ float nan_min(float a, float b) {
+  return (isnan(a) || isnan(b)) ? __int_as_float(0x7fc00000) : fminf(a, b);
+}
+__device__ __forceinline__ float nan_max(float a, float b) {
+  return (isnan(a) || isnan(b)) ? __int_as_float(0x7fc00000) : fmaxf(a, b);
+}
+
+// Tiles t in [0, nt) with !(bmax < t*ts || bmin > min((t+1)*ts, lim)), the
+// reference's closed-interval test (vector_renderer.py:437-441).  The set is
+// an interval [lo, hi]; estimates are corrected with the exact predicate.
+__device__ __forceinline__ void tile_interval(float bmin, float bmax, int nt, int ts, int lim,
+                                              int& lo, int& hi) {
+  if (isnan(bmin) || isnan(bmax)) {  // NaN compares false: every tile passes
+    lo = 0; hi = nt - 1;
+    return;
+  }
+  if (!(bmax >= 0.f)) {
+    hi = -1;
+  } else {
+    const float q = bmax / (float)ts;
+    int t = (q >= (float)nt) ? nt - 1 : (int)q;
+    while (t >= 0 && bmax < (float)(t * ts)) --t;
+    while (t + 1 < nt && !(bmax < (float)((t + 1) * ts))) ++t;
+    hi = t;
+  }
+  if (bmin > (float)lim) {
+    lo = nt;
+  } else {
+    const float q = bmin / (float)ts;
+    int t = (q <= 0.f) ? 0 : (q >= (float)nt ? nt - 1 : (int)q);
+    while (t > 0 && !(bmin > (float)min(t * ts, lim))) --t;
+    while (t < nt && bmin > (float)min((t + 1) * ts, lim)) ++t;
+    lo = t;
+  }
+}
+
+template <typename PT>
+__global__ __launch_bounds__(256) void k_prep(
+    int N, const PT* __restrict__ means2D, const PT* __restrict__ radii,
+    const PT* __restrict__ rotations, const PT* __restrict__ opacities,
+    const PT* __restrict__ colors, const int* __restrict__ sel, const float* __restrict__ ext,
+    int T, int Ht, int Wt, int H, int W, int ts, int tilesX, int tilesY, float alpha_ub,
+    PrimRec* __restrict__ recs, int* __restrict__ counts, int4* __restrict__ ranges) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const float x = ldf(means2D, 2L * n), y = ldf(means2D, 2L * n + 1);
+  const float r = ldf(radii, n), th = ldf(rotations, n);
+  const float v = ldf(opacities, n);
+  const int t = sel[n];
+  // correctly rounded fp32 trig, shared by binning and raster
+  const float c = (float)cos((double)th), s = (float)sin((double)th);
+
+  PrimRec p;
+  p.mx = x; p.my = y; p.r = r; p.c = c; p.s = s;
+  const float tw = (float)Wt, tht = (float)Ht;
+  const float md = fmaxf(tw, tht);
+  const float sf = r / (md * 0.5f);
+  p.hw = tw * 0.5f * sf;
+  p.hh = tht * 0.5f * sf;
+  p.rinv = r > 1e-6f ? 1.f / r : 1e6f;
+  p.sop = sigmoid_safe(v);
+  p.abase = alpha_ub * p.sop;
+  p.cr = sigmoid_safe(ldf(colors, 3L * n));
+  p.cg = sigmoid_safe(ldf(colors, 3L * n + 1));
+  p.cb = sigmoid_safe(ldf(colors, 3L * n + 2));
+  p.pad0 = 0; p.pad1 = 0;
+  const bool valid = (t >= 0 && t < T);
+  p.tmpl = valid ? t : -1;
+  recs[n] = p;
+
+  int cnt = 0;
+  int4 rg = make_int4(0, -1, 0, -1);
+  if (valid && ext != nullptr) {   // ext == nullptr: raster constants only
+    const float* e = ext + 4 * t;
+    const float cu[4] = {e[0], e[1], e[0], e[1]};
+    const float cv[4] = {e[2], e[2], e[3], e[3]};
+    float xmn = 0.f, xmx = 0.f, ymn = 0.f, ymx = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float a = cu[i] * c, b = cv[i] * s;
+      const float d = a - b;
+      const float wx = x + r * d;
+      const float a2 = cu[i] * s, b2 = cv[i] * c;
+      const float d2 = a2 + b2;
+      const float wy = y + r * d2;
+      if (i == 0) { xmn = xmx = wx; ymn = ymx = wy; }
+      else {
+        xmn = nan_min(xmn, wx); xmx = nan_max(xmx, wx);
+        ymn = nan_min(ymn, wy); ymx = nan_max(ymx, wy);
+      }
+    }
+    int tx0, tx1, ty0, ty1;
+    tile_interval(xmn, xmx, tilesX, ts, W, tx0, tx1);
+    tile_interval(ymn, ymx, tilesY, ts, H, ty0, ty1);
+    if (tx1 >= tx0 && ty1 >= ty0) {
+      cnt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+      rg = make_int4(tx0, tx1, ty0, ty1);
+    }
+  }
+  counts[n] = cnt;
+  ranges[n] = rg;
+}
+
+// Inclusive wave scan + block exclusive scan over 256 threads.
+__device__ __forceinline__ long long wave_incl_scan_i64(long long x) {
+  const int l = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long y = __shfl_up(x, o, 64);
+    if (l >= o) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ long long block256_excl_scan(long long v, long long& total,
+                                                        long long* sh) {
+  const int w = threadIdx.x >> 6, l = lane_id();
+  const long long inc = wave_incl_scan_i64(v);
+  if (l == 63) sh[w] = inc;
+  __syncthreads();
+  long long before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long long s = sh[i];
+    before += (i < w) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return inc - v + before;
+}
+
+// offs[i] = sum(counts[0..i)), offs[N] = *total = sum(counts).
+__global__ __launch_bounds__(256) void k_scan_counts(const int* __restrict__ counts, int N,
+                                                     long long* __restrict__ offs,
+                                                     long long* __restrict__ total) {
+  __shared__ long long sh[4];
+  constexpr int kItems = 8;
+  long long carry = 0;
+  for (int base = 0; base < N; base += 256 * kItems) {
+    const int i0 = base + threadIdx.x * kItems;
+    long long loc[kItems];
+    long long s = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      loc[j] = (i0 + j < N) ? (long long)counts[i0 + j] : 0;
+      s += loc[j];
+    }
+    long long tot;
+    long long ex = block256_excl_scan(s, tot, sh) + carry;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      if (i0 + j < N) offs[i0 + j] = ex;
+      ex += loc[j];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    offs[N] = carry;
+    *total = carry;
+  }
+}
+
+// One wave per primitive; pairs land at offs[n] in row-major tile order.
+__global__ __launch_bounds__(256) void k_emit(int N, const int* __restrict__ counts,
+                                              const long long* __restrict__ offs,
+                                              const int4* __restrict__ ranges, int tilesX,
+                                              const long long* __restrict__ total, long long cap,
+                                              int* __restrict__ keys, int* __restrict__ vals) {
+  if (*total > cap) return;
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (n >= N) return;
+  const int cnt = counts[n];
+  if (cnt == 0) return;
+  const long long off = offs[n];
+  const int4 rg = ranges[n];
+  const int nx = rg.y - rg.x + 1;
+  for (int i = lane_id(); i < cnt; i += 64) {
+    const int qy = i / nx, qx = i - qy * nx;
+    keys[off + i] = (rg.z + qy) * tilesX + rg.x + qx;
+    vals[off + i] = n;
+  }
+}
+
+__device__ __forceinline__ long long pairs_in_range(const long long* total, long long cap) {
+  const long long P = *total;
+  return P > cap ? 0 : P;
+}
+
+// Per-block digit histogram, stored digit-major: hist[d * nblk + b].
+__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(
+    const int* __restrict__ keys, const long long* __restrict__ total, long long cap, int shift,
+    int bits, int nblk, int* __restrict__ hist) {
+  __shared__ int h[256];
+  const int radix = 1 << bits, mask = radix - 1;
+  const long long P = pairs_in_range(total, cap);
+  for (int d = threadIdx.x; d < radix; d += kRadixThreads) h[d] = 0;
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kRadixTile + (threadIdx.x >> 6) * (64 * kRadixItems) + lane_id();
+#pragma unroll 4
+  for (int i = 0; i < kRadixItems; ++i) {
+    const long long idx = base + i * 64;
+    if (idx < P) atomicAdd(&h[(keys[idx] >> shift) & mask], 1);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < radix; d += kRadixThreads) hist[(long)d * nblk + blockIdx.x] = h[d];
+}
+
+// One block per digit: exclusive scan of that digit's row over blocks.
+__global__ __launch_bounds__(256) void k_radix_rowscan(int* __restrict__ hist, int nblk,
+                                                       int* __restrict__ totals) {
+  __shared__ long long sh[4];
+  int* row = hist + (long)blockIdx.x * nblk;
+  long long carry = 0;
+  for (int base = 0; base < nblk; base += 256) {
+    const int i = base + threadIdx.x;
+    const long long v = i < nblk ? row[i] : 0;
+    long long tot;
+    const long long ex = block256_excl_scan(v, tot, sh) + carry;
+    if (i < nblk) row[i] = (int)ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = (int)carry;
+}
+
+// Stable scatter.  Keys of block b are visited as wave w, item i, lane l
+// (index b*4096 + w*1024 + i*64 + l), i.e. in input order; each wave ranks its
+// items by digit with ballots and a per-wave running count in LDS.
+__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
+    const int* __restrict__ kin, const int* __restrict__ vin, int* __restrict__ kout,
+    int* __restrict__ vout, const long long* __restrict__ total, long long cap, int shift,
+    int bits, int nblk, const int* __restrict__ hist, const int* __restrict__ totals) {
+  __shared__ int dbase[256];
+  __shared__ int wcnt[4][256];
+  __shared__ long long sh[4];
+  const int radix = 1 << bits, mask = radix - 1;
+  const long long P = pairs_in_range(total, cap);
+  const int w = threadIdx.x >> 6, l = lane_id();
+  {
+    const int d = threadIdx.x;
+    const long long v = d < radix ? totals[d] : 0;
+    long long tot;
+    const long long ex = block256_excl_scan(v, tot, sh);
+    if (d < radix) dbase[d] = (int)ex;
+  }
+  for (int d = l; d < radix; d += 64) wcnt[w][d] = 0;
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kRadixTile + w * (64 * kRadixItems) + l;
+  const unsigned long long lt = (1ull << l) - 1ull;
+  int kk[kRadixItems], vv[kRadixItems], rk[kRadixItems];
+#pragma unroll
+  for (int i = 0; i < kRadixItems; ++i) {
+    const long long idx = base + i * 64;
+    const bool ok = idx < P;
+    kk[i] = ok ? kin[idx] : 0;
+    vv[i] = ok ? vin[idx] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < kRadixItems; ++i) {
+    const bool ok = base + i * 64 < P;
+    const int d = (kk[i] >> shift) & mask;
+    unsigned long long peers = __ballot(ok);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (d >> b) & 1;
+      const unsigned long long m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    int r = 0;
+    if (ok) {
+      const int cur = wcnt[w][d];
+      const unsigned long long below = peers & lt;
+      r = cur + __popcll(below);
+      if (below == 0ull) wcnt[w][d] = cur + __popcll(peers);
+    }
+    rk[i] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < radix) {
+    const int d = threadIdx.x;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = wcnt[q][d];
+      wcnt[q][d] = s;
+      s += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kRadixItems; ++i) {
+    if (base + i * 64 < P) {
+      const int d = (kk[i] >> shift) & mask;
+      const long long pos = (long long)dbase[d] + hist[(long)d * nblk + blockIdx.x] + wcnt[w][d] + rk[i];
+      kout[pos] = kk[i];
+      vout[pos] = vv[i];
+    }
+  }
+}
+
+// tile_offsets[t] = first index whose tile id >= t; tile_offsets[tiles] = P.
+__global__ __launch_bounds__(256) void k_tile_ranges(const int* __restrict__ keys,
+                                                     const long long* __restrict__ total,
+                                                     long long cap, int tiles,
+                                                     int* __restrict__ tile_offsets) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > tiles) return;
+  const long long P = pairs_in_range(total, cap);
+  long long lo = 0, hi = P;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (keys[mid] < t) lo = mid + 1; else hi = mid;
+  }
+  tile_offsets[t] = (int)lo;
+}
+
+}  // namespace dbmp

@@ -1,0 +1,550 @@
+// extern "C" entry points of libdiffbmp_amd.so (declared in include/diffbmp_amd.h).
+//
+// Host side of the MI355X rasterizer: argument checks, per-handle scratch,
+// launch sequencing on the caller's stream.  No host synchronisation, no
+// device->host reads; sizes that depend on data (the pair count) stay on the
+// device and kernels read them there.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/diffbmp_amd.h"
+#include "bin_kernels.h"
+#include "raster_kernels.h"
+
+using namespace dbmp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(DBMP_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                                          \
+  do {                                                                              \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess)                                                           \
+      return fail(DBMP_E_HIP, "launch of %s failed: %s", name, hipGetErrorString(e_)); \
+  } while (0)
+
+int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+__global__ void k_fill_f32(float* p, long long n, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <int DT>
+__global__ void k_build_quads(const void* src, int T, int Ht, int Wt, void* dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long tot = (long long)T * Ht * Wt;
+  if (i >= tot) return;
+  const int x = (int)(i % Wt);
+  const int y = (int)((i / Wt) % Ht);
+  const bool rx = x + 1 < Wt, ry = y + 1 < Ht;
+  if constexpr (DT == kF32) {
+    const float* s = (const float*)src;
+    float4 q;
+    q.x = s[i];
+    q.y = rx ? s[i + 1] : 0.f;
+    q.z = ry ? s[i + Wt] : 0.f;
+    q.w = (rx && ry) ? s[i + Wt + 1] : 0.f;
+    ((float4*)dst)[i] = q;
+  } else {
+    const unsigned short* s = (const unsigned short*)src;   // raw fp16 / bf16 bits
+    const uint32_t a = s[i], b = rx ? s[i + 1] : 0u, c = ry ? s[i + Wt] : 0u,
+                   d = (rx && ry) ? s[i + Wt + 1] : 0u;
+    ((uint2*)dst)[i] = make_uint2(a | (b << 16), c | (d << 16));
+  }
+}
+
+__global__ void k_unpack_grads(const float* __restrict__ g8, int N, float* __restrict__ gm,
+                               float* __restrict__ gr, float* __restrict__ gt,
+                               float* __restrict__ gv, float* __restrict__ gc) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const float4 a = ((const float4*)g8)[2 * n];
+  const float4 b = ((const float4*)g8)[2 * n + 1];
+  gm[2 * n] = a.x;
+  gm[2 * n + 1] = a.y;
+  gr[n] = a.z;
+  gt[n] = a.w;
+  gv[n] = b.x;
+  gc[3 * n] = b.y;
+  gc[3 * n + 1] = b.z;
+  gc[3 * n + 2] = b.w;
+}
+
+}  // namespace
+
+struct dbmp_rasterizer {
+  int H = 0, W = 0, ts = 16, tilesX = 0, tilesY = 0, tiles = 0, subs = 1;
+  float alpha_ub = 1.f;
+  int device = 0;
+  // primitive-sized scratch
+  int n_cap = 0;
+  PrimRec* recs = nullptr;
+  int* counts = nullptr;
+  long long* offs = nullptr;
+  int4* ranges = nullptr;
+  float* grad8 = nullptr;
+  // pair-sized scratch
+  long long pair_cap = 0;
+  int* keysA = nullptr;
+  int* keysB = nullptr;
+  int* valsB = nullptr;
+  int hist_blocks = 0;
+  int* hist = nullptr;
+  int* totals = nullptr;
+  // texel quads cache
+  void* quads = nullptr;
+  size_t quads_bytes = 0;
+  const void* q_src = nullptr;
+  long long q_version = -2;
+  int q_T = 0, q_Ht = 0, q_Wt = 0, q_dt = -1;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_fwd, ev_bwd;
+  double fwd_ms = 0.0, bwd_ms = 0.0;
+  long long fwd_n = 0, bwd_n = 0;
+};
+
+namespace {
+
+template <typename T>
+int grow(T** p, long long n) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  if (n <= 0) return 0;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(DBMP_E_ALLOC, "hipMalloc of %lld bytes failed: %s", (long long)(sizeof(T) * n),
+                hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int ensure_n(dbmp_rasterizer* h, int N) {
+  if (N <= h->n_cap) return 0;
+  const int cap = std::max(N, h->n_cap + h->n_cap / 4);
+  int rc;
+  if ((rc = grow(&h->recs, cap))) return rc;
+  if ((rc = grow(&h->counts, cap))) return rc;
+  if ((rc = grow(&h->offs, (long long)cap + 1))) return rc;
+  if ((rc = grow(&h->ranges, cap))) return rc;
+  if ((rc = grow(&h->grad8, 8LL * cap))) return rc;
+  h->n_cap = cap;
+  return 0;
+}
+
+int ensure_pairs(dbmp_rasterizer* h, long long cap) {
+  if (cap <= h->pair_cap) return 0;
+  int rc;
+  if ((rc = grow(&h->keysA, cap))) return rc;
+  if ((rc = grow(&h->keysB, cap))) return rc;
+  if ((rc = grow(&h->valsB, cap))) return rc;
+  const int nblk = ceil_div(cap, kRadixTile);
+  if ((rc = grow(&h->hist, 256LL * nblk))) return rc;
+  if (!h->totals && (rc = grow(&h->totals, 256))) return rc;
+  h->pair_cap = cap;
+  h->hist_blocks = nblk;
+  return 0;
+}
+
+int check_prims(const dbmp_prims* p) {
+  if (!p) return fail(DBMP_E_ARG, "prims is NULL");
+  if (p->num_primitives < 0) return fail(DBMP_E_ARG, "num_primitives < 0");
+  if (p->dtype < DBMP_F32 || p->dtype > DBMP_BF16) return fail(DBMP_E_ARG, "bad prims dtype %d", p->dtype);
+  if (p->num_primitives > 0 && (!p->means2D || !p->radii || !p->rotations || !p->opacities ||
+                                !p->colors || !p->bmp_sel))
+    return fail(DBMP_E_ARG, "a primitive parameter pointer is NULL");
+  return 0;
+}
+
+int check_bitmaps(const dbmp_bitmaps* b) {
+  if (!b) return fail(DBMP_E_ARG, "bitmaps is NULL");
+  if (!b->templates) return fail(DBMP_E_ARG, "templates is NULL");
+  if (b->num_templates <= 0 || b->height < 2 || b->width < 2)
+    return fail(DBMP_E_ARG, "templates must be (T>0, H>=2, W>=2), got (%d,%d,%d)",
+                b->num_templates, b->height, b->width);
+  if (b->dtype < DBMP_F32 || b->dtype > DBMP_BF16) return fail(DBMP_E_ARG, "bad bitmap dtype %d", b->dtype);
+  if (b->colors_orig_mode < 0 || b->colors_orig_mode > 2)
+    return fail(DBMP_E_ARG, "bad colors_orig_mode %d", b->colors_orig_mode);
+  if (b->colors_orig_mode != 0 && !b->colors_orig) return fail(DBMP_E_ARG, "colors_orig is NULL");
+  return 0;
+}
+
+int launch_prep(dbmp_rasterizer* h, const dbmp_prims* p, const float* ext, int T, int Ht, int Wt,
+                hipStream_t st) {
+  const int N = p->num_primitives;
+  if (N == 0) return 0;
+  const dim3 grid(ceil_div(N, 256)), block(256);
+#define PREP_ARGS(TY)                                                                          \
+  N, (const TY*)p->means2D, (const TY*)p->radii, (const TY*)p->rotations,                      \
+      (const TY*)p->opacities, (const TY*)p->colors, p->bmp_sel, ext, T, Ht, Wt, h->H, h->W,   \
+      h->ts, h->tilesX, h->tilesY, h->alpha_ub, h->recs, h->counts, h->ranges
+  if (p->dtype == DBMP_F32)
+    hipLaunchKernelGGL(k_prep<float>, grid, block, 0, st, PREP_ARGS(float));
+  else if (p->dtype == DBMP_F16)
+    hipLaunchKernelGGL(k_prep<__half>, grid, block, 0, st, PREP_ARGS(__half));
+  else
+    hipLaunchKernelGGL(k_prep<__hip_bfloat16>, grid, block, 0, st, PREP_ARGS(__hip_bfloat16));
+#undef PREP_ARGS
+  LAUNCH_CHECK("k_prep");
+  return 0;
+}
+
+int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
+  const long long tot = (long long)b->num_templates * b->height * b->width;
+  const size_t esz = b->dtype == DBMP_F32 ? 16 : 8;
+  if (b->version >= 0 && h->q_src == b->templates && h->q_version == b->version &&
+      h->q_T == b->num_templates && h->q_Ht == b->height && h->q_Wt == b->width &&
+      h->q_dt == b->dtype)
+    return 0;
+  if ((size_t)tot * esz > h->quads_bytes) {
+    if (h->quads) (void)hipFree(h->quads);
+    h->quads = nullptr;
+    h->quads_bytes = 0;
+    hipError_t e = hipMalloc(&h->quads, (size_t)tot * esz);
+    if (e != hipSuccess) return fail(DBMP_E_ALLOC, "hipMalloc quads: %s", hipGetErrorString(e));
+    h->quads_bytes = (size_t)tot * esz;
+  }
+  const dim3 grid(ceil_div(tot, 256)), block(256);
+  if (b->dtype == DBMP_F32)
+    hipLaunchKernelGGL(k_build_quads<kF32>, grid, block, 0, st, b->templates, b->num_templates,
+                       b->height, b->width, h->quads);
+  else
+    hipLaunchKernelGGL(k_build_quads<kF16>, grid, block, 0, st, b->templates, b->num_templates,
+                       b->height, b->width, h->quads);
+  LAUNCH_CHECK("k_build_quads");
+  h->q_src = b->templates;
+  h->q_version = b->version;
+  h->q_T = b->num_templates;
+  h->q_Ht = b->height;
+  h->q_Wt = b->width;
+  h->q_dt = b->dtype;
+  return 0;
+}
+
+RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps* b,
+                     const dbmp_tile_lists* L, float eps) {
+  RasterArgs A;
+  A.recs = h->recs;
+  A.toffs = L->tile_offsets;
+  A.tidx = L->tile_indices;
+  A.quads = h->quads;
+  const bool blend = b->colors_orig_mode != 0 && b->c_blend > 0.f;
+  A.cmap = blend ? b->colors_orig : nullptr;
+  A.cmap_per_prim = b->colors_orig_mode == 1;
+  A.c_blend = b->c_blend;
+  A.N = p->num_primitives;
+  A.Ht = b->height;
+  A.Wt = b->width;
+  A.Wtm1 = (float)(b->width - 1);
+  A.Htm1 = (float)(b->height - 1);
+  A.hsx = 0.5f * (float)(b->width - 1);
+  A.hsy = 0.5f * (float)(b->height - 1);
+  A.H = h->H;
+  A.W = h->W;
+  A.ts = h->ts;
+  A.tilesX = h->tilesX;
+  A.tiles = h->tiles;
+  A.subs = h->subs;
+  A.eps = eps;
+  A.alpha_ub = h->alpha_ub;
+  return A;
+}
+
+void timing_begin(dbmp_rasterizer* h, std::vector<std::pair<hipEvent_t, hipEvent_t>>& v,
+                  hipStream_t st) {
+  if (!h->timing) return;
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return;
+  if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+  (void)hipEventRecord(a, st);
+  v.emplace_back(a, b);
+}
+
+void timing_end(dbmp_rasterizer* h, std::vector<std::pair<hipEvent_t, hipEvent_t>>& v,
+                hipStream_t st) {
+  if (!h->timing || v.empty()) return;
+  (void)hipEventRecord(v.back().second, st);
+}
+
+void timing_drain(std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, double& ms, long long& n) {
+  for (auto& e : v) {
+    float t = 0.f;
+    if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&t, e.first, e.second) == hipSuccess) {
+      ms += t;
+      ++n;
+    }
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  v.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int dbmp_version(void) { return 100; }
+
+const char* dbmp_last_error(void) { return g_err.c_str(); }
+
+int dbmp_init(dbmp_rasterizer** out, int image_height, int image_width, int tile_size, float sigma,
+              float alpha_upper_bound, int max_prims_per_pixel, int num_primitives) {
+  (void)sigma;
+  (void)max_prims_per_pixel;
+  if (!out) return fail(DBMP_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (image_height <= 0 || image_width <= 0)
+    return fail(DBMP_E_ARG, "image size must be positive, got %dx%d", image_height, image_width);
+  if (tile_size <= 0 || tile_size % kSub != 0)
+    return fail(DBMP_E_ARG, "tile_size must be a positive multiple of %d, got %d", kSub, tile_size);
+  auto* h = new dbmp_rasterizer();
+  h->H = image_height;
+  h->W = image_width;
+  h->ts = tile_size;
+  h->tilesX = (image_width + tile_size - 1) / tile_size;
+  h->tilesY = (image_height + tile_size - 1) / tile_size;
+  h->tiles = h->tilesX * h->tilesY;
+  h->subs = (tile_size / kSub) * (tile_size / kSub);
+  h->alpha_ub = alpha_upper_bound;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return fail(DBMP_E_HIP, "hipGetDevice failed");
+  }
+  if (num_primitives > 0) {
+    int rc = ensure_n(h, num_primitives);
+    if (rc) {
+      dbmp_destroy(h);
+      return rc;
+    }
+  }
+  *out = h;
+  return 0;
+}
+
+int dbmp_destroy(dbmp_rasterizer* h) {
+  if (!h) return 0;
+  double ms = 0;
+  long long n = 0;
+  timing_drain(h->ev_fwd, ms, n);
+  timing_drain(h->ev_bwd, ms, n);
+  void* ptrs[] = {h->recs, h->counts, h->offs, h->ranges, h->grad8, h->keysA,
+                  h->keysB, h->valsB, h->hist, h->totals, h->quads};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete h;
+  return 0;
+}
+
+int dbmp_template_extents(dbmp_rasterizer* h, const dbmp_bitmaps* bm, float* ext_out, void* stream) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  int rc = check_bitmaps(bm);
+  if (rc) return rc;
+  if (!ext_out) return fail(DBMP_E_ARG, "ext_out is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid(bm->num_templates), block(256);
+  if (bm->dtype == DBMP_F32)
+    hipLaunchKernelGGL(k_template_extents<float>, grid, block, 0, st, (const float*)bm->templates,
+                       bm->height, bm->width, ext_out);
+  else if (bm->dtype == DBMP_F16)
+    hipLaunchKernelGGL(k_template_extents<__half>, grid, block, 0, st, (const __half*)bm->templates,
+                       bm->height, bm->width, ext_out);
+  else
+    hipLaunchKernelGGL(k_template_extents<__hip_bfloat16>, grid, block, 0, st,
+                       (const __hip_bfloat16*)bm->templates, bm->height, bm->width, ext_out);
+  LAUNCH_CHECK("k_template_extents");
+  return 0;
+}
+
+int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
+             int tmpl_h, int tmpl_w, int32_t* tile_offsets, int32_t* tile_indices,
+             int64_t capacity, int64_t* d_pair_count, void* stream) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  int rc = check_prims(prims);
+  if (rc) return rc;
+  if (!ext || !tile_offsets || !d_pair_count) return fail(DBMP_E_ARG, "NULL output/extents pointer");
+  if (num_templates <= 0 || tmpl_h < 2 || tmpl_w < 2) return fail(DBMP_E_ARG, "bad template shape");
+  if (capacity < 0 || capacity > (int64_t)0x7fffffff)
+    return fail(DBMP_E_ARG, "capacity must be in [0, 2^31), got %lld", (long long)capacity);
+  if (capacity > 0 && !tile_indices) return fail(DBMP_E_ARG, "tile_indices is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = prims->num_primitives;
+  if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
+  if ((rc = ensure_pairs(h, std::max<long long>(capacity, 1)))) return rc;
+  long long* dtot = (long long*)d_pair_count;
+  if ((rc = launch_prep(h, prims, ext, num_templates, tmpl_h, tmpl_w, st))) return rc;
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(256), 0, st, h->counts, N, h->offs, dtot);
+  LAUNCH_CHECK("k_scan_counts");
+
+  int bits = 0;
+  while ((1 << bits) < h->tiles) ++bits;
+  const int passes = (bits + 7) / 8;
+  const int dbits = passes ? (bits + passes - 1) / passes : 0;
+  const bool even = (passes % 2) == 0;
+  int* kcur = h->keysA;
+  int* vcur = even ? tile_indices : h->valsB;
+  int* kalt = h->keysB;
+  int* valt = even ? h->valsB : tile_indices;
+  if (N > 0 && capacity > 0) {
+    hipLaunchKernelGGL(k_emit, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, h->counts, h->offs,
+                       h->ranges, h->tilesX, dtot, (long long)capacity, kcur, vcur);
+    LAUNCH_CHECK("k_emit");
+    const int nblk = ceil_div(capacity, kRadixTile);
+    for (int p = 0; p < passes; ++p) {
+      const int shift = p * dbits;
+      const int nb = std::min(dbits, bits - shift);
+      hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, dtot,
+                         (long long)capacity, shift, nb, nblk, h->hist);
+      LAUNCH_CHECK("k_radix_hist");
+      hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << nb), dim3(256), 0, st, h->hist, nblk, h->totals);
+      LAUNCH_CHECK("k_radix_rowscan");
+      hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, vcur, kalt,
+                         valt, dtot, (long long)capacity, shift, nb, nblk, h->hist, h->totals);
+      LAUNCH_CHECK("k_radix_scatter");
+      std::swap(kcur, kalt);
+      std::swap(vcur, valt);
+    }
+  }
+  hipLaunchKernelGGL(k_tile_ranges, dim3(ceil_div(h->tiles + 1, 256)), dim3(256), 0, st, kcur,
+                     dtot, (long long)capacity, h->tiles, tile_offsets);
+  LAUNCH_CHECK("k_tile_ranges");
+  return 0;
+}
+
+#define DISPATCH_DT_BLEND(DT, BLEND, KERNEL, ...)                                            \
+  do {                                                                                       \
+    if ((DT) == DBMP_F32) {                                                                  \
+      if (BLEND) hipLaunchKernelGGL((KERNEL<kF32, true>), __VA_ARGS__);                      \
+      else hipLaunchKernelGGL((KERNEL<kF32, false>), __VA_ARGS__);                           \
+    } else if ((DT) == DBMP_F16) {                                                           \
+      if (BLEND) hipLaunchKernelGGL((KERNEL<kF16, true>), __VA_ARGS__);                      \
+      else hipLaunchKernelGGL((KERNEL<kF16, false>), __VA_ARGS__);                           \
+    } else {                                                                                 \
+      if (BLEND) hipLaunchKernelGGL((KERNEL<kBF16, true>), __VA_ARGS__);                     \
+      else hipLaunchKernelGGL((KERNEL<kBF16, false>), __VA_ARGS__);                          \
+    }                                                                                        \
+  } while (0)
+
+int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                 const dbmp_tile_lists* lists, float eps_T, float* out_color, float* out_alpha,
+                 float* out_T_final, int32_t* out_last, void* stream) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  int rc = check_prims(prims);
+  if (rc) return rc;
+  if ((rc = check_bitmaps(bm))) return rc;
+  if (!lists || !lists->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  if (!out_color || !out_alpha || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL output");
+  if (!(eps_T >= 0.f)) return fail(DBMP_E_ARG, "eps_T must be >= 0");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = prims->num_primitives;
+  if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
+  timing_begin(h, h->ev_fwd, st);
+  if ((rc = ensure_quads(h, bm, st))) return rc;
+  // raster constants only (no extents => no tile ranges)
+  if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
+  const RasterArgs A = make_args(h, prims, bm, lists, eps_T);
+  const bool blend = A.cmap != nullptr;
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(h->tiles * h->subs), dim3(64), 0, st, A,
+                    out_color, out_alpha, out_T_final, out_last);
+  LAUNCH_CHECK("k_raster_fwd");
+  timing_end(h, h->ev_fwd, st);
+  return 0;
+}
+
+int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                  const dbmp_tile_lists* lists, float eps_T, const float* grad_color,
+                  const float* grad_alpha, const float* out_color, const float* out_T_final,
+                  const int32_t* out_last, float* grad_means2D, float* grad_radii,
+                  float* grad_rotations, float* grad_opacities, float* grad_colors, void* stream) {
+  (void)eps_T;
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  int rc = check_prims(prims);
+  if (rc) return rc;
+  if ((rc = check_bitmaps(bm))) return rc;
+  if (!lists || !lists->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  if (!grad_color || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
+  if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
+    return fail(DBMP_E_ARG, "NULL gradient output");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = prims->num_primitives;
+  if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
+  timing_begin(h, h->ev_bwd, st);
+  if ((rc = ensure_quads(h, bm, st))) return rc;
+  if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
+  if (N > 0) {
+    hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
+    LAUNCH_CHECK("k_fill_f32");
+  }
+  const RasterArgs A = make_args(h, prims, bm, lists, 0.f);
+  const bool blend = A.cmap != nullptr;
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd, dim3(h->tiles * h->subs), dim3(64), 0, st, A,
+                    grad_color, grad_alpha, out_color, out_T_final, out_last, h->grad8);
+  LAUNCH_CHECK("k_raster_bwd");
+  if (N > 0) {
+    hipLaunchKernelGGL(k_unpack_grads, dim3(ceil_div(N, 256)), dim3(256), 0, st, h->grad8, N,
+                       grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);
+    LAUNCH_CHECK("k_unpack_grads");
+  }
+  timing_end(h, h->ev_bwd, st);
+  return 0;
+}
+
+int dbmp_timing_enable(dbmp_rasterizer* h, int on) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  h->timing = on != 0;
+  return 0;
+}
+
+int dbmp_timing_stats(dbmp_rasterizer* h, double* fwd_ms_total, int64_t* fwd_count,
+                      double* bwd_ms_total, int64_t* bwd_count) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  long long fn = h->fwd_n, bn = h->bwd_n;
+  timing_drain(h->ev_fwd, h->fwd_ms, fn);
+  timing_drain(h->ev_bwd, h->bwd_ms, bn);
+  h->fwd_n = fn;
+  h->bwd_n = bn;
+  if (fwd_ms_total) *fwd_ms_total = h->fwd_ms;
+  if (fwd_count) *fwd_count = h->fwd_n;
+  if (bwd_ms_total) *bwd_ms_total = h->bwd_ms;
+  if (bwd_count) *bwd_count = h->bwd_n;
+  return 0;
+}
+
+int dbmp_timing_reset(dbmp_rasterizer* h) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  double ms = 0;
+  long long n = 0;
+  timing_drain(h->ev_fwd, ms, n);
+  timing_drain(h->ev_bwd, ms, n);
+  h->fwd_ms = h->bwd_ms = 0.0;
+  h->fwd_n = h->bwd_n = 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+// Device-side building blocks shared by the binning and raster kernels.
+//
+// Compiled with -ffp-contract=off: the per-sample geometry (push box, texture
+// coordinates, floor) must round every fp32 operation exactly as written so
+// that its discrete decisions are reproducible on the host.  Value arithmetic
+// (mask interpolation, compositing, gradients) uses explicit fmaf.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+namespace dbmp {
+
+enum DType : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
+
+// Per-primitive constants, computed once per call by the prep kernel and
+// read by the raster kernels through uniform (scalar) loads.  64 bytes.
+struct alignas(16) PrimRec {
+  float mx, my, hw, hh;      // centre; push-box half extents (tile_forward.cu:195-210)
+  float c, s, rinv, abase;   // cos, sin (fp32, correctly rounded), 1/r, alpha_ub * sigmoid(v)
+  float cr, cg, cb, sop;     // sigmoid(colour logits), sigmoid(opacity logit)
+  int tmpl;                  // template index, -1 => never pushed
+  float r;                   // radius
+  int pad0, pad1;
+};
+static_assert(sizeof(PrimRec) == 64, "PrimRec is 64 bytes");
+
+// Storage-type loads -------------------------------------------------------
+__device__ __forceinline__ float ldf(const float* p, long i) { return p[i]; }
+__device__ __forceinline__ float ldf(const __half* p, long i) { return __half2float(p[i]); }
+__device__ __forceinline__ float ldf(const __hip_bfloat16* p, long i) { return __bfloat162float(p[i]); }
+
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ __forceinline__ float f16_lo(uint32_t w) {
+  return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu)));
+}
+__device__ __forceinline__ float f16_hi(uint32_t w) {
+  return __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+}
+
+// Texel quads: entry (y,x) of template t holds its bilinear neighbourhood
+// {T[y][x], T[y][x+1], T[y+1][x], T[y+1][x+1]}, zero beyond the last row or
+// column (the reference's zero padding, tile_common.h:188-194).  One vector
+// load per sample instead of four gathers.
+template <int DT> struct QuadT;
+template <> struct QuadT<kF32> {
+  using type = float4;
+  __device__ static __forceinline__ void get(const float4* q, long i, float& a, float& b, float& c, float& d) {
+    float4 v = q[i];
+    a = v.x; b = v.y; c = v.z; d = v.w;
+  }
+};
+template <> struct QuadT<kF16> {
+  using type = uint2;
+  __device__ static __forceinline__ void get(const uint2* q, long i, float& a, float& b, float& c, float& d) {
+    uint2 v = q[i];
+    a = f16_lo(v.x); b = f16_hi(v.x); c = f16_lo(v.y); d = f16_hi(v.y);
+  }
+};
+template <> struct QuadT<kBF16> {
+  using type = uint2;
+  __device__ static __forceinline__ void get(const uint2* q, long i, float& a, float& b, float& c, float& d) {
+    uint2 v = q[i];
+    a = bf16_lo(v.x); b = bf16_hi(v.x); c = bf16_lo(v.y); d = bf16_hi(v.y);
+  }
+};
+
+// sigmoidf_safe (tile_common.h:125-133); evaluated once per primitive
+__device__ __forceinline__ float sigmoid_safe(float x) {
+  if (x >= 0.f) {
+    float z = expf(-x);
+    return 1.f / (1.f + z);
+  }
+  float z = expf(x);
+  return z / (1.f + z);
+}
+
+// Per-sample geometry in exact fp32 (tile_forward.cu:187-267).  Returns
+// false when the pixel is outside the push box or the texture coordinates
+// fall outside [0, W-1] x [0, H-1] (bilinear_sample returns 0 there, and a
+// zero mask is never pushed).
+struct Geo {
+  float u, v, ndx, ndy, tx, ty, wx, wy;
+  int x0, y0;
+};
+
+__device__ __forceinline__ bool sample_geom(const PrimRec& p, float px, float py, float Wtm1,
+                                            float Htm1, float hsx, float hsy, int Wt, int Ht,
+                                            Geo& g) {
+  const float dx = px - p.mx, dy = py - p.my;
+  if (fabsf(dx) > p.hw || fabsf(dy) > p.hh) return false;
+  const float ndx = dx * p.rinv, ndy = dy * p.rinv;
+  const float u = p.c * ndx + p.s * ndy;
+  const float v = -p.s * ndx + p.c * ndy;
+  // (u + 1) * 0.5 * (W-1) == (u + 1) * (0.5 * (W-1)) bit for bit: the 0.5
+  // scaling is exact on both sides.
+  const float tx = (u + 1.f) * hsx;
+  const float ty = (v + 1.f) * hsy;
+  if (tx < 0.f || tx > Wtm1 || ty < 0.f || ty > Htm1) return false;
+  const float fx = floorf(tx), fy = floorf(ty);
+  g.x0 = min(max((int)fx, 0), Wt - 1);
+  g.y0 = min(max((int)fy, 0), Ht - 1);
+  g.wx = tx - fx;
+  g.wy = ty - fy;
+  g.u = u; g.v = v; g.ndx = ndx; g.ndy = ndy; g.tx = tx; g.ty = ty;
+  return true;
+}
+
+// Bilinear value and texel-space gradient from a quad (tile_common.h:201-239)
+__device__ __forceinline__ float bilerp(float v00, float v01, float v10, float v11, float wx,
+                                        float wy) {
+  const float top = fmaf(wx, v01 - v00, v00);
+  const float bot = fmaf(wx, v11 - v10, v10);
+  return fmaf(wy, bot - top, top);
+}
+
+__device__ __forceinline__ float bilerp_grad(float v00, float v01, float v10, float v11, float wx,
+                                             float wy, float& gxt, float& gyt) {
+  const float dxa = v01 - v00, dxb = v11 - v10;
+  const float top = fmaf(wx, dxa, v00);
+  const float bot = fmaf(wx, dxb, v10);
+  gxt = fmaf(wy, dxb - dxa, dxa);
+  gyt = bot - top;
+  return fmaf(wy, bot - top, top);
+}
+
+// Clamp-to-edge bilinear RGB colour map (tile_forward.cu:17-89); the
+// coordinates are already inside [0,W-1] x [0,H-1] for a pushed sample.
+template <typename CT>
+__device__ __forceinline__ void color_map_sample(const CT* cm, int Ht, int Wt, float cx, float cy,
+                                                 float out[3]) {
+  cx = fmaxf(0.f, fminf((float)(Wt - 1), cx));
+  cy = fmaxf(0.f, fminf((float)(Ht - 1), cy));
+  const int x0 = (int)cx, y0 = (int)cy;
+  const int x1 = min(x0 + 1, Wt - 1), y1 = min(y0 + 1, Ht - 1);
+  const float fx = cx - (float)x0, fy = cy - (float)y0;
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    const float c00 = ldf(cm, 3L * (y0 * Wt + x0) + ch);
+    const float c01 = ldf(cm, 3L * (y1 * Wt + x0) + ch);
+    const float c10 = ldf(cm, 3L * (y0 * Wt + x1) + ch);
+    const float c11 = ldf(cm, 3L * (y1 * Wt + x1) + ch);
+    const float c0 = fmaf(fx, c10 - c00, c00);
+    const float c1 = fmaf(fx, c11 - c01, c01);
+    out[ch] = fmaf(fy, c1 - c0, c0);
+  }
+}
+
+// Wave-level helpers (wave64) ---------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Reduce-scatter of eight per-lane values: afterwards every lane of the
+// 8-lane group g = lane >> 3 holds the wave total of value index
+// ((lane>>5)&1)*4 + ((lane>>4)&1)*2 + ((lane>>3)&1).  10 shuffles instead of
+// 48 for eight independent butterflies.
+__device__ __forceinline__ float wave_reduce_scatter8(const float v[8], int& which) {
+  const int l = lane_id();
+  const bool b5 = (l >> 5) & 1, b4 = (l >> 4) & 1, b3 = (l >> 3) & 1;
+  float w[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float mine = b5 ? v[4 + i] : v[i];
+    const float other = b5 ? v[i] : v[4 + i];
+    w[i] = mine + __shfl_xor(other, 32, 64);
+  }
+  float x[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const float mine = b4 ? w[2 + i] : w[i];
+    const float other = b4 ? w[i] : w[2 + i];
+    x[i] = mine + __shfl_xor(other, 16, 64);
+  }
+  float y = (b3 ? x[1] : x[0]) + __shfl_xor(b3 ? x[0] : x[1], 8, 64);
+  y += __shfl_xor(y, 4, 64);
+  y += __shfl_xor(y, 2, 64);
+  y += __shfl_xor(y, 1, 64);
+  which = (b5 ? 4 : 0) + (b4 ? 2 : 0) + (b3 ? 1 : 0);
+  return y;
+}
+
+}  // namespace dbmp

@@ -1,0 +1,374 @@
+"""Reference-compatible autograd boundary of the MI355X tile rasterizer.
+
+Mirrors ``cuda_tile_rasterizer/__init__.py`` (``TileRasterizer`` :60-121,
+``TileRasterizerBatchFunction`` :124-252, ``TileRasterizerFunction`` :254-340,
+``TileRasterizerFunctionFP16`` :342-436, timing helpers :439-465): same class
+and function names, argument order and meaning, 16-argument ``apply`` with a
+16-entry backward.  A renderer written against the reference switches by
+importing these names from ``diffbmp_amd.tile_rasterizer``.
+
+Differences a caller can observe, all deliberate:
+
+* forward -> backward state (per-pixel ``T_final`` and list position) is
+  kept in ``ctx``, not in a process-wide cache, so a second forward before the
+  first backward is safe (the reference's batch backward reads the last
+  candidate's cache for every candidate, ``tile_rasterize.cu:533-628``);
+* ``tile_primitive_mapping=None`` (or empty) bins on the GPU instead of reusing
+  the previous mapping (``tile_rasterize.cu:323``);
+* ``colors_orig`` is read only when ``c_blend > 0`` (as the kernel does) and is
+  not cast or copied otherwise;
+* FP16 storage uses fp32 transmittance / colour accumulation and fp32
+  gradients (the reference accumulates in half, ``tile_forward_fp16.cu:373-401``);
+  outputs are returned as fp16 like the reference;
+* a pixel stops compositing once its transmittance is below ``eps_T``
+  (``DEFAULT_EPS_T``); the dropped colour is below that bound.  Set it to 0 to
+  walk every list entry as the reference does;
+* errors raise (``NativeError`` / ``RuntimeError``); there is no silent
+  fallback to a different renderer.
+"""
+from __future__ import annotations
+
+import threading
+
+import torch
+from torch.autograd import Function
+
+from . import _native
+from ._native import Handle
+
+__all__ = [
+    "TileRasterizer", "TileRasterizerFunction", "TileRasterizerFunctionFP16",
+    "TileRasterizerBatchFunction", "CUDA_AVAILABLE", "CUDA_FP16_AVAILABLE",
+    "print_cuda_timing_stats", "reset_cuda_timing_stats", "print_cuda_timing_stats_fp16",
+    "reset_cuda_timing_stats_fp16", "DEFAULT_EPS_T", "bin_primitives", "pack_mapping",
+    "unpack_mapping",
+]
+
+DEFAULT_EPS_T = 1e-6
+
+
+def _native_available() -> bool:
+    try:
+        _native.lib()
+        return True
+    except Exception:
+        return False
+
+
+# Names kept from the reference: "CUDA" here is torch's device namespace,
+# which PyTorch-ROCm maps to HIP.
+CUDA_AVAILABLE = _native_available()
+CUDA_FP16_AVAILABLE = CUDA_AVAILABLE
+
+# The active rasterizer per precision, set by TileRasterizer.__init__ like the
+# reference's init_tile_rasterizer (ext.cpp:5-8).  Only configuration lives
+# here; per-call state travels in autograd ctx.
+_ACTIVE: dict[str, Handle] = {}
+_ACTIVE_LOCK = threading.Lock()
+
+
+def _active(kind: str) -> Handle:
+    h = _ACTIVE.get(kind)
+    if h is None:
+        raise RuntimeError("TileRasterizer not initialized. Call init_tile_rasterizer first.")
+    return h
+
+
+def pack_mapping(tile_offsets: torch.Tensor, tile_indices: torch.Tensor) -> torch.Tensor:
+    """Reference wire format ``[tiles+1, P, offsets..., indices...]`` int32
+    (``simple_tile_renderer.py:757-766``)."""
+    hdr = torch.tensor([tile_offsets.numel(), tile_indices.numel()], dtype=torch.int32,
+                       device=tile_offsets.device)
+    return torch.cat([hdr, tile_offsets.to(torch.int32), tile_indices.to(torch.int32)])
+
+
+def unpack_mapping(mapping: torch.Tensor, tiles: int):
+    """Split a packed mapping without reading it back: the offsets block has
+    ``tiles + 1`` entries for this canvas and tile size."""
+    if mapping.dtype != torch.int32 or mapping.dim() != 1:
+        raise TypeError("tile_primitive_mapping must be a 1-D int32 tensor")
+    if mapping.numel() < tiles + 3:
+        raise ValueError(f"tile_primitive_mapping has {mapping.numel()} entries, "
+                         f"expected at least {tiles + 3} for {tiles} tiles")
+    return mapping[2:3 + tiles], mapping[3 + tiles:]
+
+
+def bin_primitives(handle: Handle, means2D, radii, rotations, bmp_sel, templates,
+                   extents: torch.Tensor | None = None):
+    """GPU binning: ``(tile_offsets, tile_indices)`` for the given primitives,
+    identical to the reference's ``_unified_primitive_assignment`` +
+    ``_convert_masks_to_mapping`` (``vector_renderer.py:401-444``,
+    ``simple_tile_renderer.py:721-766``)."""
+    dt = means2D.dtype
+    zeros = torch.zeros_like(radii)
+    prims = Handle.prims(means2D.contiguous(), radii.contiguous(), rotations.contiguous(),
+                         zeros, torch.zeros((radii.shape[0], 3), dtype=dt, device=radii.device),
+                         bmp_sel.to(torch.int32).contiguous())
+    if extents is None:
+        extents = handle.template_extents(templates)
+    return handle.bin(prims, extents, tuple(templates.shape))
+
+
+class _Run:
+    """Everything one forward needs, normalised to contiguous device tensors."""
+
+    def __init__(self, handle: Handle, store_dtype, means2D, radii, rotations, opacities, colors,
+                 colors_orig, templates, bmp_sel, c_blend, mapping, eps_T,
+                 colors_orig_per_template=False, tmpl_version=-1):
+        for t, nm in ((means2D, "means2D"), (templates, "primitive_templates")):
+            _native.require_gpu(t, nm)
+        cv = lambda t: t.to(dtype=store_dtype).contiguous()
+        self.means2D, self.radii, self.rotations = cv(means2D), cv(radii), cv(rotations)
+        self.opacities, self.colors = cv(opacities), cv(colors)
+        self.templates = cv(templates)
+        self.tmpl_version = tmpl_version if self.templates.data_ptr() == templates.data_ptr() else -1
+        self.sel = bmp_sel.to(torch.int32).contiguous()
+        if isinstance(c_blend, torch.Tensor):
+            c_blend = float(c_blend.item())
+        self.c_blend = float(c_blend)
+        self.colors_orig, self.co_mode = None, 0
+        if colors_orig is not None and self.c_blend > 0.0 and colors_orig.numel() > 0:
+            self.co_mode = 2 if colors_orig_per_template else 1
+            need = self.templates.shape[0] if self.co_mode == 2 else self.radii.shape[0]
+            if colors_orig.dim() != 4 or colors_orig.shape[0] < need or \
+                    tuple(colors_orig.shape[1:]) != (*self.templates.shape[1:], 3):
+                raise ValueError(f"colors_orig must be ({need},Ht,Wt,3), got {tuple(colors_orig.shape)}")
+            self.colors_orig = cv(colors_orig)
+        self.eps_T = float(eps_T)
+        self.handle = handle
+        if mapping is not None and mapping.numel() > 0:
+            self.offsets, self.indices = unpack_mapping(mapping, handle.tiles)
+        else:
+            ext = handle.template_extents(self.templates)
+            self.offsets, self.indices = handle.bin(self.prims(), ext, tuple(self.templates.shape))
+
+    def prims(self):
+        return Handle.prims(self.means2D, self.radii, self.rotations, self.opacities, self.colors,
+                            self.sel)
+
+    def bitmaps(self):
+        return Handle.bitmaps(self.templates, self.colors_orig, self.co_mode, self.c_blend,
+                              self.tmpl_version)
+
+    def forward(self):
+        dev, H, W = self.means2D.device, self.handle.H, self.handle.W
+        out_color = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+        out_alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
+        out_T = torch.empty((H, W), dtype=torch.float32, device=dev)
+        out_last = torch.empty((H, W), dtype=torch.int32, device=dev)
+        self.handle.forward(self.prims(), self.bitmaps(), self.offsets, self.indices, self.eps_T,
+                            out_color, out_alpha, out_T, out_last)
+        return out_color, out_alpha, out_T, out_last
+
+    def backward(self, grad_color, grad_alpha, out_color, out_T, out_last):
+        dev, N = self.means2D.device, self.radii.shape[0]
+        f32 = torch.float32
+        gC = grad_color.to(f32).contiguous()
+        gA = None if grad_alpha is None else grad_alpha.to(f32).contiguous()
+        g_means = torch.empty((N, 2), dtype=f32, device=dev)
+        g_r = torch.empty(N, dtype=f32, device=dev)
+        g_th = torch.empty(N, dtype=f32, device=dev)
+        g_v = torch.empty(N, dtype=f32, device=dev)
+        g_c = torch.empty((N, 3), dtype=f32, device=dev)
+        self.handle.backward(self.prims(), self.bitmaps(), self.offsets, self.indices, self.eps_T,
+                             gC, gA, out_color, out_T, out_last, g_means, g_r, g_th, g_v, g_c)
+        return g_means, g_r, g_th, g_v, g_c
+
+
+def _apply_forward(ctx, kind, store_dtype, out_dtype, means2D, radii, rotations, opacities, colors,
+                   colors_orig, primitive_templates, global_bmp_sel, c_blend, lr_conf,
+                   tile_primitive_mapping, image_height, image_width, tile_size, sigma):
+    handle = _active(kind)
+    if (int(image_height), int(image_width), int(tile_size)) != (handle.H, handle.W, handle.ts):
+        raise ValueError(f"rasterizer was initialised for {handle.H}x{handle.W} tile {handle.ts}, "
+                         f"called with {image_height}x{image_width} tile {tile_size}")
+    run = _Run(handle, store_dtype, means2D, radii, rotations, opacities, colors, colors_orig,
+               primitive_templates, global_bmp_sel, c_blend, tile_primitive_mapping,
+               getattr(handle, "eps_T", DEFAULT_EPS_T), tmpl_version=primitive_templates._version)
+    out_color, out_alpha, out_T, out_last = run.forward()
+    ctx.run = run
+    ctx.save_for_backward(out_color, out_T, out_last)
+    ctx.mark_non_differentiable(out_T, out_last)
+    if out_dtype != torch.float32:
+        return out_color.to(out_dtype), out_alpha.to(out_dtype)
+    return out_color, out_alpha
+
+
+def _apply_backward(ctx, grad_out_color, grad_out_alpha):
+    out_color, out_T, out_last = ctx.saved_tensors
+    run = ctx.run
+    if grad_out_color is None:
+        grad_out_color = torch.zeros_like(out_color)
+    grads = run.backward(grad_out_color, grad_out_alpha, out_color, out_T, out_last)
+    return tuple(grads) + (None,) * 11
+
+
+class TileRasterizerFunction(Function):
+    """FP32 forward/backward (``cuda_tile_rasterizer/__init__.py:254-340``)."""
+
+    @staticmethod
+    def forward(ctx, means2D, radii, rotations, opacities, colors, colors_orig, primitive_templates,
+                global_bmp_sel, c_blend, lr_conf, tile_primitive_mapping, image_height, image_width,
+                tile_size, sigma, use_class=False):
+        return _apply_forward(ctx, "fp32", torch.float32, torch.float32, means2D, radii, rotations,
+                              opacities, colors, colors_orig, primitive_templates, global_bmp_sel,
+                              c_blend, lr_conf, tile_primitive_mapping, image_height, image_width,
+                              tile_size, sigma)
+
+    @staticmethod
+    def backward(ctx, grad_out_color, grad_out_alpha):
+        return _apply_backward(ctx, grad_out_color, grad_out_alpha)
+
+
+class TileRasterizerFunctionFP16(Function):
+    """FP16-storage forward/backward (``cuda_tile_rasterizer/__init__.py:342-436``):
+    parameters and bitmaps are stored as fp16, arithmetic and accumulation are
+    fp32, outputs are returned as fp16."""
+
+    @staticmethod
+    def forward(ctx, means2D, radii, rotations, opacities, colors, colors_orig, primitive_templates,
+                global_bmp_sel, c_blend, lr_conf, tile_primitive_mapping, image_height, image_width,
+                tile_size, sigma, use_class=False):
+        return _apply_forward(ctx, "fp16", torch.float16, torch.float16, means2D, radii, rotations,
+                              opacities, colors, colors_orig, primitive_templates, global_bmp_sel,
+                              c_blend, lr_conf, tile_primitive_mapping, image_height, image_width,
+                              tile_size, sigma)
+
+    @staticmethod
+    def backward(ctx, grad_out_color, grad_out_alpha):
+        return _apply_backward(ctx, grad_out_color, grad_out_alpha)
+
+
+class TileRasterizerBatchFunction(Function):
+    """B candidates with shared templates (``cuda_tile_rasterizer/__init__.py:124-252``).
+    Each candidate keeps its own tile lists and forward state, so every
+    candidate's gradient is its own."""
+
+    @staticmethod
+    def forward(ctx, means2D, radii, rotations, opacities, colors, colors_orig, primitive_templates,
+                global_bmp_sel, c_blend, lr_conf, tile_primitive_mappings, image_height, image_width,
+                tile_size, sigma, use_fp16=False):
+        kind = "fp16" if use_fp16 else "fp32"
+        handle = _active(kind)
+        store = torch.float16 if use_fp16 else torch.float32
+        B = means2D.shape[0]
+        maps = list(tile_primitive_mappings) if tile_primitive_mappings else []
+        runs, outs_c, outs_a, saved = [], [], [], []
+        for b in range(B):
+            mp = maps[b] if b < len(maps) else None
+            co = None
+            if colors_orig is not None and colors_orig.numel() > 0:
+                co = colors_orig[b] if colors_orig.dim() == 5 else colors_orig
+            run = _Run(handle, store, means2D[b], radii[b], rotations[b], opacities[b], colors[b], co,
+                       primitive_templates, global_bmp_sel, c_blend, mp,
+                       getattr(handle, "eps_T", DEFAULT_EPS_T),
+                       tmpl_version=primitive_templates._version)
+            oc, oa, oT, ol = run.forward()
+            runs.append(run)
+            outs_c.append(oc)
+            outs_a.append(oa)
+            saved += [oc, oT, ol]
+        ctx.runs = runs
+        ctx.save_for_backward(*saved)
+        out_c, out_a = torch.stack(outs_c), torch.stack(outs_a)
+        if use_fp16:
+            return out_c.half(), out_a.half()
+        return out_c, out_a
+
+    @staticmethod
+    def backward(ctx, grad_out_color, grad_out_alpha):
+        saved = ctx.saved_tensors
+        gs = [[] for _ in range(5)]
+        for b, run in enumerate(ctx.runs):
+            oc, oT, ol = saved[3 * b: 3 * b + 3]
+            gC = grad_out_color[b] if grad_out_color is not None else torch.zeros_like(oc)
+            gA = grad_out_alpha[b] if grad_out_alpha is not None else None
+            for i, g in enumerate(run.backward(gC, gA, oc, oT, ol)):
+                gs[i].append(g)
+        return tuple(torch.stack(g) for g in gs) + (None,) * 11
+
+
+class TileRasterizer:
+    """Class-based rasterizer (``cuda_tile_rasterizer/__init__.py:60-121``)."""
+
+    def __init__(self, image_height, image_width, tile_size=16, sigma=0.0, alpha_upper_bound=1.0,
+                 max_prims_per_pixel=151, num_primitives=None, use_fp16=False):
+        self.image_height = image_height
+        self.image_width = image_width
+        self.tile_size = tile_size
+        self.sigma = sigma
+        self.alpha_upper_bound = alpha_upper_bound
+        self.max_prims_per_pixel = max_prims_per_pixel
+        self.num_primitives = num_primitives
+        self.use_fp16 = use_fp16
+        kind = "fp16" if use_fp16 else "fp32"
+        self.handle = Handle(image_height, image_width, tile_size, sigma, alpha_upper_bound,
+                             max_prims_per_pixel, num_primitives)
+        self.handle.eps_T = DEFAULT_EPS_T
+        with _ACTIVE_LOCK:
+            _ACTIVE[kind] = self.handle
+        self.apply_fn = TileRasterizerFunctionFP16.apply if use_fp16 else TileRasterizerFunction.apply
+
+    def __call__(self, means2D, radii, rotations, opacities, colors, colors_orig, primitive_templates,
+                 global_bmp_sel, c_blend, lr_conf, tile_primitive_mapping=None):
+        with _ACTIVE_LOCK:
+            _ACTIVE["fp16" if self.use_fp16 else "fp32"] = self.handle
+        return self.apply_fn(means2D, radii, rotations, opacities, colors, colors_orig,
+                             primitive_templates, global_bmp_sel, c_blend, lr_conf,
+                             tile_primitive_mapping, self.image_height, self.image_width,
+                             self.tile_size, self.sigma, True)
+
+    def forward_batch(self, means2D, radii, rotations, opacities, colors, colors_orig,
+                      primitive_templates, global_bmp_sel, c_blend, lr_conf,
+                      tile_primitive_mappings=None):
+        with _ACTIVE_LOCK:
+            _ACTIVE["fp16" if self.use_fp16 else "fp32"] = self.handle
+        return TileRasterizerBatchFunction.apply(
+            means2D, radii, rotations, opacities, colors, colors_orig, primitive_templates,
+            global_bmp_sel, c_blend, lr_conf, tile_primitive_mappings, self.image_height,
+            self.image_width, self.tile_size, self.sigma, self.use_fp16)
+
+    @property
+    def eps_T(self) -> float:
+        """Early-termination threshold on transmittance (0 = walk every entry)."""
+        return self.handle.eps_T
+
+    @eps_T.setter
+    def eps_T(self, v: float) -> None:
+        self.handle.eps_T = float(v)
+
+    def enable_timing(self, on: bool = True) -> None:
+        self.handle.timing_enable(on)
+
+
+def _print_stats(kind: str) -> None:
+    h = _ACTIVE.get(kind)
+    if h is None:
+        print("Rasterizer not initialized: no timing statistics")
+        return
+    f, fn, b, bn = h.timing_stats()
+    print("=== tile rasterizer timing (%s, hipEvent) ===" % kind)
+    print("forward : %d calls, total %.3f ms, avg %.4f ms" % (fn, f, f / fn if fn else 0.0))
+    print("backward: %d calls, total %.3f ms, avg %.4f ms" % (bn, b, b / bn if bn else 0.0))
+
+
+def print_cuda_timing_stats():
+    """Average forward/backward time (``tile_rasterize.cu:783-806``); timing is
+    recorded only after ``TileRasterizer.enable_timing()``."""
+    _print_stats("fp32")
+
+
+def reset_cuda_timing_stats():
+    h = _ACTIVE.get("fp32")
+    if h is not None:
+        h.timing_reset()
+
+
+def print_cuda_timing_stats_fp16():
+    _print_stats("fp16")
+
+
+def reset_cuda_timing_stats_fp16():
+    h = _ACTIVE.get("fp16")
+    if h is not None:
+        h.timing_reset()

@@ -1,0 +1,139 @@
+/*
+ * diffbmp_amd.h -- C ABI of the MI355X (gfx950) DiffBMP tile rasterizer.
+ *
+ * The library replaces the reference's CUDA torch extension `_C` / `_C_fp16`
+ * (cuda_tile_rasterizer/ext.cpp:5-121, ext_fp16.cpp:5-82) on the
+ * binning -> forward -> backward path.  Every entry point takes plain device
+ * pointers, sizes and a hipStream_t (as void*), never synchronises the host,
+ * never reads tensors back, and returns 0 on success or a nonzero DBMP_E*
+ * code, with a message from dbmp_last_error().  State lives in a handle
+ * (not a process-wide singleton): forward->backward state is returned to the
+ * caller in caller-owned buffers (T_final, last), so several forward passes can
+ * be outstanding at once.  A handle's internal scratch (primitive records,
+ * sort buffers, gradient accumulator, texel quads) is shared by its calls:
+ * use one handle per stream.
+ *
+ * Reference conventions kept: means2D (N,2) [x,y] in pixels with integer pixel
+ * centres; radii (N) half extent in pixels; rotations (N) radians; opacities
+ * (N) and colors (N,3) are logits; templates (T,Ht,Wt) alpha in [0,1];
+ * bmp_sel (N) int32 template index; colour output premultiplied over black,
+ * alpha = 1 - prod(1 - a_k); tile lists row-major by tile, ascending primitive
+ * id inside a tile (cuda_tile_rasterizer/__init__.py:84-92,
+ * pydiffbmp/core/renderer/simple_tile_renderer.py:721-766).
+ */
+#ifndef DIFFBMP_AMD_H
+#define DIFFBMP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DBMP_F32 0
+#define DBMP_F16 1
+#define DBMP_BF16 2
+
+#define DBMP_OK 0
+#define DBMP_E_ARG 1      /* invalid argument / unsupported shape */
+#define DBMP_E_HIP 2      /* HIP runtime or launch error */
+#define DBMP_E_ALLOC 3    /* device allocation failed */
+
+typedef struct dbmp_rasterizer dbmp_rasterizer;
+
+/* Per-primitive parameters, storage dtype DBMP_F32 / F16 / BF16. */
+typedef struct {
+  const void* means2D;     /* (N,2) */
+  const void* radii;       /* (N)   */
+  const void* rotations;   /* (N)   */
+  const void* opacities;   /* (N) logits */
+  const void* colors;      /* (N,3) logits */
+  const int32_t* bmp_sel;  /* (N) template index */
+  int32_t num_primitives;
+  int32_t dtype;
+} dbmp_prims;
+
+/* Alpha bitmaps and optional colour maps (same dtype as the bitmaps). */
+typedef struct {
+  const void* templates;    /* (T,Ht,Wt) */
+  const void* colors_orig;  /* (N,Ht,Wt,3) or (T,Ht,Wt,3) in [0,1]; may be NULL */
+  int32_t num_templates, height, width;
+  int32_t dtype;
+  int32_t colors_orig_mode; /* 0: none, 1: per primitive (N,..), 2: per template (T,..) */
+  float c_blend;            /* colour-map blend, used when > 0 and colors_orig set */
+  int64_t version;          /* caller's content version of `templates`; -1 = unknown */
+} dbmp_bitmaps;
+
+/* CSR tile lists: tile_offsets (tiles+1), tile_indices (P). */
+typedef struct {
+  const int32_t* tile_offsets;
+  const int32_t* tile_indices;
+} dbmp_tile_lists;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dbmp_version(void);
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* dbmp_last_error(void);
+
+/* Replaces _C.init_tile_rasterizer (ext.cpp:5-8, tile_rasterize.cu:15-207).
+ * tile_size must be a positive multiple of 16.  sigma and
+ * max_prims_per_pixel are accepted for signature parity and not used: the
+ * reference never reads sigma in its kernels, and this implementation keeps
+ * no per-pixel layer cache, so it has no per-pixel capacity limit. */
+int dbmp_init(dbmp_rasterizer** out, int image_height, int image_width, int tile_size,
+              float sigma, float alpha_upper_bound, int max_prims_per_pixel, int num_primitives);
+
+/* Frees the handle and its scratch. */
+int dbmp_destroy(dbmp_rasterizer* h);
+
+/* Template extents (T,4) fp32 (min_u,max_u,min_v,max_v) of texels > 1e-6;
+ * replaces VectorRenderer._compute_primitive_bboxes
+ * (pydiffbmp/core/renderer/vector_renderer.py:151-192). */
+int dbmp_template_extents(dbmp_rasterizer* h, const dbmp_bitmaps* bm, float* ext_out, void* stream);
+
+/* Binning: tile lists for the primitives; replaces the reference's torch
+ * binning and packing (vector_renderer.py:194-252,401-444;
+ * simple_tile_renderer.py:479-516,721-766).  `ext` is (T,4) from
+ * dbmp_template_extents.  Writes the pair count P to *d_pair_count (device
+ * int64).  If P > capacity nothing else is written except tile_offsets, which
+ * is set to all zeros (empty lists); the caller grows its buffer and calls
+ * again.  tile_indices must hold `capacity` int32. */
+int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
+             int tmpl_h, int tmpl_w, int32_t* tile_offsets, int32_t* tile_indices,
+             int64_t capacity, int64_t* d_pair_count, void* stream);
+
+/* Forward; replaces _C.rasterize_tiles_class / _C_fp16.rasterize_tiles_class_fp16
+ * (ext.cpp:11-28 -> tile_rasterize.cu:297-498).  Outputs (fp32):
+ *   out_color (H,W,3) premultiplied over black, out_alpha (H,W),
+ *   out_T_final (H,W) and out_last (H,W) int32 = tile-list entries consumed,
+ *   which dbmp_backward needs.  eps_T: a pixel stops compositing once its
+ *   transmittance drops below eps_T (0 = walk the whole list). */
+int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                 const dbmp_tile_lists* lists, float eps_T, float* out_color, float* out_alpha,
+                 float* out_T_final, int32_t* out_last, void* stream);
+
+/* Backward; replaces _C.rasterize_tiles_backward_class (ext.cpp:31-50 ->
+ * tile_rasterize.cu:638-780).  grad_color (H,W,3), grad_alpha (H,W) or NULL;
+ * out_color / out_T_final / out_last are dbmp_forward's outputs for the same
+ * inputs.  Gradients (fp32) are overwritten: grad_means2D (N,2), grad_radii
+ * (N), grad_rotations (N), grad_opacities (N), grad_colors (N,3). */
+int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                  const dbmp_tile_lists* lists, float eps_T, const float* grad_color,
+                  const float* grad_alpha, const float* out_color, const float* out_T_final,
+                  const int32_t* out_last, float* grad_means2D, float* grad_radii,
+                  float* grad_rotations, float* grad_opacities, float* grad_colors, void* stream);
+
+/* Timing: hipEvent brackets around each forward/backward launch sequence;
+ * replaces print_/reset_cuda_timing_stats (ext.cpp:110-121,
+ * tile_rasterize.cu:783-831).  dbmp_timing_stats synchronises the recorded
+ * events. */
+int dbmp_timing_enable(dbmp_rasterizer* h, int on);
+int dbmp_timing_stats(dbmp_rasterizer* h, double* fwd_ms_total, int64_t* fwd_count,
+                      double* bwd_ms_total, int64_t* bwd_count);
+int dbmp_timing_reset(dbmp_rasterizer* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIFFBMP_AMD_H */

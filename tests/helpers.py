@@ -1,0 +1,86 @@
+"""Shared test utilities: scene -> device tensors, oracle comparisons."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import oracle as O
+from diffbmp_amd import _native
+from diffbmp_amd._native import Handle
+
+
+def to_dev(scene, dev, dtype=torch.float32):
+    t = lambda a, dt=dtype: torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(dt)
+    return dict(means2D=t(scene.means2D), radii=t(scene.radii), rotations=t(scene.rotations),
+                opacities=t(scene.opacities), colors=t(scene.colors),
+                templates=t(scene.templates), bmp_sel=t(scene.bmp_sel, torch.int32))
+
+
+def rounded(scene, dtype):
+    """Scene whose float arrays are rounded to `dtype` storage and back to fp32."""
+    import copy
+    s = copy.copy(scene)
+    r = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dtype).float().numpy()
+    s.means2D, s.radii, s.rotations = r(scene.means2D), r(scene.radii), r(scene.rotations)
+    s.opacities, s.colors, s.templates = r(scene.opacities), r(scene.colors), r(scene.templates)
+    if scene.colors_orig_tmpl is not None:
+        s.colors_orig_tmpl = r(scene.colors_orig_tmpl)
+    return s
+
+
+class GpuRun:
+    """Bins, renders and back-propagates one scene through the C ABI."""
+
+    def __init__(self, scene, dev, dtype=torch.float32, eps_T=0.0, ts=16, co_mode=0):
+        self.scene, self.dev, self.dtype, self.eps = scene, dev, dtype, eps_T
+        self.h = Handle(scene.H, scene.W, ts, 0.0, scene.alpha_ub, 151, scene.N)
+        self.d = to_dev(scene, dev, dtype)
+        d = self.d
+        self.prims = Handle.prims(d["means2D"], d["radii"], d["rotations"], d["opacities"],
+                                  d["colors"], d["bmp_sel"])
+        self.co = None
+        if co_mode:
+            src = scene.colors_orig_tmpl if co_mode == 2 else scene.colors_orig_per_prim()
+            self.co = torch.from_numpy(np.ascontiguousarray(src)).to(dev).to(dtype)
+        self.bm = Handle.bitmaps(d["templates"], self.co, co_mode, scene.c_blend, -1)
+        self.ext = self.h.template_extents(d["templates"])
+        self.offs, self.idx = self.h.bin(self.prims, self.ext, tuple(d["templates"].shape))
+
+    def forward(self):
+        H, W, dev = self.scene.H, self.scene.W, self.dev
+        self.color = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+        self.alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
+        self.T = torch.empty((H, W), dtype=torch.float32, device=dev)
+        self.last = torch.empty((H, W), dtype=torch.int32, device=dev)
+        self.h.forward(self.prims, self.bm, self.offs, self.idx, self.eps, self.color, self.alpha,
+                       self.T, self.last)
+        torch.cuda.synchronize()
+        return self.color.cpu().numpy(), self.alpha.cpu().numpy()
+
+    def backward(self, gC, gA):
+        N, dev = self.scene.N, self.dev
+        gCt = torch.from_numpy(np.ascontiguousarray(gC, np.float32)).to(dev)
+        gAt = None if gA is None else torch.from_numpy(np.ascontiguousarray(gA, np.float32)).to(dev)
+        g = dict(means2D=torch.empty((N, 2), device=dev), radii=torch.empty(N, device=dev),
+                 rotations=torch.empty(N, device=dev), opacities=torch.empty(N, device=dev),
+                 colors=torch.empty((N, 3), device=dev))
+        self.h.backward(self.prims, self.bm, self.offs, self.idx, self.eps, gCt, gAt, self.color,
+                        self.T, self.last, g["means2D"], g["radii"], g["rotations"],
+                        g["opacities"], g["colors"])
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in g.items()}
+
+
+def normwise(g, ref):
+    """Per-group ||g - ref|| / ||ref||."""
+    out = {}
+    for k in ref:
+        den = np.linalg.norm(ref[k].ravel())
+        out[k] = float(np.linalg.norm((g[k] - ref[k]).ravel()) / (den if den > 0 else 1.0))
+    return out
+
+
+def rand_grads(H, W, seed=0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((H, W, 3)) * scale).astype(np.float32), \
+           (rng.standard_normal((H, W)) * scale).astype(np.float32)
