@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: fwd+bwd iterations/s of the tile rasterizer on one MI355X.
+
+Workload (``scale``): 1024x1024 RGB canvas on a white background, 8192
+primitives over four 128x128 fp32 alpha bitmaps (soft disk, five-point star,
+ring, blurred thresholded noise), 16x16 tiles, target a smooth random field.
+One step = GPU binning + forward + background composite + MSE + backward +
+Adam (lr 1e-2) + parameter clamp -- nothing in the optimisation step is
+skipped.  ``value`` is steps/s over the K timed steps (wall clock between two
+device synchronisations); per-step hipEvent times of the fwd+bwd segment
+(binning, forward, loss, backward) are reported as p50/p99 beside it.
+
+``roofline``: the dominant kernel (backward raster) -- its compulsory HBM
+bytes per launch divided by its mean duration, measured live with hipEvents
+on the launch stream, against 8 TB/s.  ``cpu_baseline``: the reference
+algorithm in fp32 on the host cores (OpenMP build of the oracle), timed on a
+bounded sample of tiles and scaled to the full canvas.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--eps-t", type=float, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-tiles", type=int, default=48)
+    return ap.parse_args()
+
+
+def algorithmic_bytes(H, W, N, T, Ht, Wt, P, tiles, b_p, b_t):
+    """Compulsory HBM bytes per step of binning, forward and backward: every
+    input read once, every output written once (primitive parameters
+    8 values, template texels, the P tile-list entries, per-pixel output and
+    saved state, per-primitive fp32 gradients)."""
+    b_bin = N * (8 * b_p + 4) + 4 * P + 4 * (tiles + 1)
+    b_fwd = N * (8 * b_p + 4) + T * Ht * Wt * b_t + 4 * P + H * W * 16 + 8 * H * W
+    b_bwd = H * W * 16 + 12 * H * W + 8 * H * W + N * (8 * b_p + 4) + T * Ht * Wt * b_t + 4 * P + 32 * N
+    return b_bin, b_fwd, b_bwd
+
+
+def cpu_baseline(scene, ntiles_sample):
+    """The reference algorithm in fp32 (per-pixel caches, O(K^2) backward) on
+    the host's cores, on a random sample of tiles, scaled to the full canvas."""
+    import oracle as O
+    L = O.lib("f32")
+    cores = int(L.oracle_num_threads())
+    t0 = time.perf_counter()
+    ext = O.template_extents_np(scene.templates)
+    offs, idx = O.bin_lists_c(scene.means2D, scene.radii, scene.rotations, scene.bmp_sel, ext,
+                              scene.H, scene.W, scene.tile_size)
+    t_bin = time.perf_counter() - t0
+    tiles = len(offs) - 1
+    rng = np.random.default_rng(99)
+    sub = np.sort(rng.choice(tiles, min(ntiles_sample, tiles), replace=False)).astype(np.int32)
+    osc = O.scene_from(scene, offs, idx, tile_subset=sub)
+    gC = np.full((scene.H, scene.W, 3), 1e-6, np.float32)
+    gA = np.full((scene.H, scene.W), 1e-6, np.float32)
+    t0 = time.perf_counter()
+    O.forward(osc, "f32")
+    t_f = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.backward(osc, gC, gA, "f32")
+    t_b = time.perf_counter() - t0
+    scale = tiles / len(sub)
+    step_s = t_bin + (t_f + t_b) * scale
+    return dict(value=1.0 / step_s, unit="fwd+bwd iters/s", cores=cores, kind="port",
+                sample=f"{len(sub)} of {tiles} tiles (fwd+bwd, fp32 port of the reference "
+                       f"algorithm) scaled x{scale:.1f}, plus full binning; "
+                       f"{step_s * 1e3:.0f} ms per full step")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the rasterizer "
+              "path has no multi-GPU mode in this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import torch.nn.functional as F
+    from diffbmp_amd import scenes as S
+    from diffbmp_amd.tile_rasterizer import TileRasterizer, DEFAULT_EPS_T
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    sc = S.make_scene(1)
+    H, W, N = sc.H, sc.W, sc.N
+    T, Ht, Wt = sc.templates.shape
+    sdt = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[args.precision]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    means2D = t(sc.means2D).requires_grad_()
+    radii = t(sc.radii).requires_grad_()
+    rot = t(sc.rotations).requires_grad_()
+    op = t(sc.opacities).requires_grad_()
+    col = t(sc.colors).requires_grad_()
+    templates = t(sc.templates).to(sdt)
+    sel = t(sc.bmp_sel)
+    target = t(sc.target)
+    bg = torch.ones((H, W, 3), device=dev)
+    empty = torch.empty(0, device=dev)
+    lr_conf = torch.tensor([1e-2, 1, 1, 1, 1, 1, 1], device=dev)
+    rast = TileRasterizer(H, W, 16, 0.0, 1.0, 151, N, use_fp16=(args.precision != "fp32"),
+                          storage_dtype=sdt)
+    if args.eps_t is not None:
+        rast.eps_T = args.eps_t
+    opt = torch.optim.Adam([means2D, radii, rot, op, col], lr=1e-2)
+    rmax = float(min(H, W) // 4)
+
+    stream = torch.cuda.current_stream(dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def step(record):
+        opt.zero_grad(set_to_none=True)
+        if record:
+            ev[0].record(stream)
+        color, alpha = rast(means2D, radii, rot, op, col, empty, templates, sel, 0.0, lr_conf, None)
+        result = color.float() + (1.0 - alpha.float()).unsqueeze(-1) * bg
+        loss = F.mse_loss(result, target)
+        loss.backward()
+        if record:
+            ev[1].record(stream)
+        opt.step()
+        with torch.no_grad():
+            means2D[:, 0].clamp_(0.0, float(W - 1))
+            means2D[:, 1].clamp_(0.0, float(H - 1))
+            radii.clamp_(2.0, rmax)
+        return loss
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+
+    # per-kernel (raster launch) timing inside the library, hipEvent on the launch stream
+    rast.handle.timing_reset()
+    rast.handle.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rast.handle.timing_enable(False)
+    f_ms, f_n, b_ms, b_n = rast.handle.timing_stats()
+
+    # a second, shorter pass for per-step fwd+bwd latency percentiles
+    lat = []
+    for _ in range(min(args.steps, 100)):
+        step(True)
+        ev[1].synchronize()
+        lat.append(ev[0].elapsed_time(ev[1]))
+    lat = np.array(lat)
+
+    # pair count of the final state for the byte model
+    with torch.no_grad():
+        from diffbmp_amd.tile_rasterizer import bin_primitives
+        offs, idx = bin_primitives(rast.handle, means2D.detach().to(sdt), radii.detach().to(sdt),
+                                   rot.detach().to(sdt), sel, templates)
+        P = int(idx.numel())
+    b_p = torch.tensor([], dtype=sdt).element_size()
+    b_t = b_p
+    b_bin, b_fwd, b_bwd = algorithmic_bytes(H, W, N, T, Ht, Wt, P, rast.handle.tiles, b_p, b_t)
+    total = t1 - t0
+    ms_step = total / args.steps * 1e3
+    bwd_avg = b_ms / max(b_n, 1) * 1e-3
+    fwd_avg = f_ms / max(f_n, 1) * 1e-3
+    achieved = b_bwd / bwd_avg
+    out = {
+        "metric": "fwd+bwd iters/s at 1024^2, 8192 prims",
+        "value": round(args.steps / total, 3),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp16": "f16-store/f32-acc", "bf16": "bf16-store/f32-acc"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "scale: 1024x1024 RGB, 8192 primitives, 4x128^2 bitmaps, 16x16 tiles, "
+                               "Adam lr 1e-2 on MSE to a smooth random field",
+                   "canvas": [H, W], "primitives": N, "bitmaps": [T, Ht, Wt], "tile": 16,
+                   "eps_T": rast.eps_T, "precision": args.precision},
+        "fwd_bwd_ms_p50": round(float(np.percentile(lat, 50)), 4),
+        "fwd_bwd_ms_p99": round(float(np.percentile(lat, 99)), 4),
+        "raster_fwd_ms": round(fwd_avg * 1e3, 4),
+        "raster_bwd_ms": round(bwd_avg * 1e3, 4),
+        "pairs": P,
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
+                     "kernel": "k_raster_bwd", "algorithmic_bytes": int(b_bwd),
+                     "step_algorithmic_bytes": int(b_bin + b_fwd + b_bwd)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_tiles)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -179,6 +179,8 @@ def _apply_forward(ctx, kind, store_dtype, out_dtype, means2D, radii, rotations,
                    colors_orig, primitive_templates, global_bmp_sel, c_blend, lr_conf,
                    tile_primitive_mapping, image_height, image_width, tile_size, sigma):
     handle = _active(kind)
+    if kind == "fp16":   # reduced-precision storage: fp16 unless the rasterizer chose bf16
+        store_dtype = out_dtype = getattr(handle, "store_dtype", store_dtype)
     if (int(image_height), int(image_width), int(tile_size)) != (handle.H, handle.W, handle.ts):
         raise ValueError(f"rasterizer was initialised for {handle.H}x{handle.W} tile {handle.ts}, "
                          f"called with {image_height}x{image_width} tile {tile_size}")
@@ -250,7 +252,7 @@ class TileRasterizerBatchFunction(Function):
                 tile_size, sigma, use_fp16=False):
         kind = "fp16" if use_fp16 else "fp32"
         handle = _active(kind)
-        store = torch.float16 if use_fp16 else torch.float32
+        store = getattr(handle, "store_dtype", torch.float16) if use_fp16 else torch.float32
         B = means2D.shape[0]
         maps = list(tile_primitive_mappings) if tile_primitive_mappings else []
         runs, outs_c, outs_a, saved = [], [], [], []
@@ -272,7 +274,7 @@ class TileRasterizerBatchFunction(Function):
         ctx.save_for_backward(*saved)
         out_c, out_a = torch.stack(outs_c), torch.stack(outs_a)
         if use_fp16:
-            return out_c.half(), out_a.half()
+            return out_c.to(store), out_a.to(store)
         return out_c, out_a
 
     @staticmethod
@@ -292,7 +294,14 @@ class TileRasterizer:
     """Class-based rasterizer (``cuda_tile_rasterizer/__init__.py:60-121``)."""
 
     def __init__(self, image_height, image_width, tile_size=16, sigma=0.0, alpha_upper_bound=1.0,
-                 max_prims_per_pixel=151, num_primitives=None, use_fp16=False):
+                 max_prims_per_pixel=151, num_primitives=None, use_fp16=False, storage_dtype=None):
+        """``storage_dtype`` (extension): torch.float16 or torch.bfloat16 selects the
+        reduced-precision storage of the ``use_fp16`` path; torch.float32 with
+        ``use_fp16=False`` is the plain path."""
+        if storage_dtype is not None and storage_dtype != torch.float32:
+            if storage_dtype not in (torch.float16, torch.bfloat16):
+                raise ValueError(f"unsupported storage_dtype {storage_dtype}")
+            use_fp16 = True
         self.image_height = image_height
         self.image_width = image_width
         self.tile_size = tile_size
@@ -305,6 +314,8 @@ class TileRasterizer:
         self.handle = Handle(image_height, image_width, tile_size, sigma, alpha_upper_bound,
                              max_prims_per_pixel, num_primitives)
         self.handle.eps_T = DEFAULT_EPS_T
+        self.handle.store_dtype = (storage_dtype if storage_dtype is not None and use_fp16
+                                   else (torch.float16 if use_fp16 else torch.float32))
         with _ACTIVE_LOCK:
             _ACTIVE[kind] = self.handle
         self.apply_fn = TileRasterizerFunctionFP16.apply if use_fp16 else TileRasterizerFunction.apply
