@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--eps-t", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=48)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a known-size "
+                         "reference kernel for scaling HBM byte counters)")
     return ap.parse_args()
 
 
@@ -100,6 +103,13 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        src = torch.ones(nb // 4, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sc = S.make_scene(1)
     H, W, N = sc.H, sc.W, sc.N
     T, Ht, Wt = sc.templates.shape
@@ -167,12 +177,18 @@ def main():
         lat.append(ev[0].elapsed_time(ev[1]))
     lat = np.array(lat)
 
-    # pair count of the final state for the byte model
+    # pair count and list-walk depth of the final state
     with torch.no_grad():
-        from diffbmp_amd.tile_rasterizer import bin_primitives
-        offs, idx = bin_primitives(rast.handle, means2D.detach().to(sdt), radii.detach().to(sdt),
-                                   rot.detach().to(sdt), sel, templates)
-        P = int(idx.numel())
+        from diffbmp_amd.tile_rasterizer import _Run
+        run = _Run(rast.handle, sdt, means2D.detach(), radii.detach(), rot.detach(), op.detach(),
+                   col.detach(), None, templates, sel, 0.0, None, rast.eps_T)
+        _, _, _, last = run.forward()
+        torch.cuda.synchronize()
+        offs = run.lists.offsets.long()
+        P = int(offs[-1].item())
+        lens = (offs[1:] - offs[:-1]).float()
+        walked = float(last.float().mean().item())
+        list_mean = float(lens.mean().item())
     b_p = torch.tensor([], dtype=sdt).element_size()
     b_t = b_p
     b_bin, b_fwd, b_bwd = algorithmic_bytes(H, W, N, T, Ht, Wt, P, rast.handle.tiles, b_p, b_t)
@@ -203,6 +219,8 @@ def main():
         "raster_fwd_ms": round(fwd_avg * 1e3, 4),
         "raster_bwd_ms": round(bwd_avg * 1e3, 4),
         "pairs": P,
+        "list_len_mean": round(list_mean, 2),
+        "entries_walked_per_pixel": round(walked, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
                      "kernel": "k_raster_bwd", "algorithmic_bytes": int(b_bwd),

@@ -39,7 +39,24 @@ class Bitmaps(ctypes.Structure):
 
 
 class TileLists(ctypes.Structure):
-    _fields_ = [("tile_offsets", ctypes.c_void_p), ("tile_indices", ctypes.c_void_p)]
+    _fields_ = [("tile_offsets", ctypes.c_void_p), ("tile_indices", ctypes.c_void_p),
+                ("pair_count", ctypes.c_void_p), ("capacity", ctypes.c_int64)]
+
+
+class Lists:
+    """Tile lists on the device: CSR offsets, index buffer, and -- for lists
+    produced by dbmp_bin -- the device pair count and the capacity used."""
+
+    __slots__ = ("offsets", "indices", "count", "capacity")
+
+    def __init__(self, offsets, indices, count=None, capacity=0):
+        self.offsets, self.indices, self.count, self.capacity = offsets, indices, count, int(capacity)
+
+    def c(self) -> TileLists:
+        idx = self.indices if self.indices.numel() else self.offsets
+        return TileLists(self.offsets.data_ptr(), idx.data_ptr(),
+                         self.count.data_ptr() if self.count is not None else None,
+                         self.capacity)
 
 
 EXPORTS = {
@@ -140,6 +157,9 @@ class Handle:
         self._h = h
         self._L = L
         self.pair_capacity = 0
+        self._pending = None          # (event, pinned count) of the last async binning
+        self._tmpl_cache = None       # (key, converted templates, extents, version)
+        self._tmpl_gen = 0
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -223,17 +243,64 @@ class Handle:
                 raise RuntimeError(f"{P} (tile, primitive) pairs exceed the int32 index range")
         raise RuntimeError("binning did not converge")  # unreachable
 
-    def forward(self, prims: Prims, bm: Bitmaps, offsets: torch.Tensor, indices: torch.Tensor,
-                eps_T: float, out_color, out_alpha, out_T, out_last) -> None:
-        lists = TileLists(offsets.data_ptr(), indices.data_ptr() if indices.numel() else offsets.data_ptr())
-        check(self._L.dbmp_forward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(lists),
+    def prepared_templates(self, templates: torch.Tensor, store_dtype):
+        """Templates converted to the storage dtype, their extents and a content
+        version, cached while the source tensor is unchanged (same storage,
+        same torch version counter)."""
+        key = (templates.data_ptr(), templates._version, tuple(templates.shape), templates.dtype,
+               store_dtype, templates.device)
+        c = self._tmpl_cache
+        if c is not None and c[0] == key:
+            return c[1], c[2], c[3]
+        conv = templates.to(dtype=store_dtype).contiguous()
+        if conv.data_ptr() == templates.data_ptr():
+            conv = conv.clone()        # own the bytes the cache keys on
+        ext = self.template_extents(conv)
+        self._tmpl_gen += 1
+        self._tmpl_cache = (key, conv, ext, self._tmpl_gen)
+        return conv, ext, self._tmpl_gen
+
+    def bin_lazy(self, prims: Prims, ext: torch.Tensor, tmpl_shape) -> Lists:
+        """Tile lists without a host synchronisation in steady state.  The
+        capacity follows the pair counts seen so far (read back
+        asynchronously); the first call, with no history, reads the count
+        once.  If a call overflows, the raster kernels walk every primitive for
+        that call (exact, slower) and the next call uses a larger buffer."""
+        dev = ext.device
+        N = prims.num_primitives
+        if self._pending is not None and self._pending[0].query():
+            seen = int(self._pending[1][0])
+            if seen > 0.9 * self.pair_capacity:
+                self.pair_capacity = int(seen * 1.3) + 4096
+            self._pending = None
+        if self.pair_capacity == 0:
+            offs, idx = self.bin(prims, ext, tmpl_shape)
+            self.pair_capacity = int(idx.numel() * 1.3) + 4096
+            return Lists(offs, idx)
+        cap = self.pair_capacity
+        offs = torch.empty(self.tiles + 1, dtype=torch.int32, device=dev)
+        idx = torch.empty(cap, dtype=torch.int32, device=dev)
+        cnt = torch.empty(1, dtype=torch.int64, device=dev)
+        self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt)
+        if self._pending is None:
+            host = torch.empty(1, dtype=torch.int64, pin_memory=True)
+            host.copy_(cnt, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(torch.cuda.current_stream(dev))
+            self._pending = (ev, host)
+        return Lists(offs, idx, cnt, cap)
+
+    def forward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T: float, out_color, out_alpha,
+                out_T, out_last) -> None:
+        L = lists.c()
+        check(self._L.dbmp_forward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(L),
                                    float(eps_T), _ptr(out_color), _ptr(out_alpha), _ptr(out_T),
                                    _ptr(out_last), _stream(out_color.device)))
 
-    def backward(self, prims: Prims, bm: Bitmaps, offsets, indices, eps_T, grad_color, grad_alpha,
+    def backward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T, grad_color, grad_alpha,
                  out_color, out_T, out_last, g_means, g_r, g_th, g_v, g_c) -> None:
-        lists = TileLists(offsets.data_ptr(), indices.data_ptr() if indices.numel() else offsets.data_ptr())
-        check(self._L.dbmp_backward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(lists),
+        L = lists.c()
+        check(self._L.dbmp_backward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(L),
                                     float(eps_T), _ptr(grad_color), _ptr(grad_alpha), _ptr(out_color),
                                     _ptr(out_T), _ptr(out_last), _ptr(g_means), _ptr(g_r), _ptr(g_th),
                                     _ptr(g_v), _ptr(g_c), _stream(grad_color.device)))

@@ -11,7 +11,7 @@
 //                 pairs, tiles in row-major order, primitives in id order.
 //   k_radix_*     stable LSD radix sort on the tile id; stability keeps the
 //                 primitive ids ascending inside each tile.
-//   k_tile_ranges CSR offsets by binary search over the sorted tile ids.
+//   k_tile_bounds CSR offsets from the boundaries of the sorted tile ids.
 #pragma once
 #include "dbmp_device.h"
 
@@ -120,9 +120,14 @@ __global__ __launch_bounds__(256) void k_prep(
   p.cr = sigmoid_safe(ldf(colors, 3L * n));
   p.cg = sigmoid_safe(ldf(colors, 3L * n + 1));
   p.cb = sigmoid_safe(ldf(colors, 3L * n + 2));
-  p.pad0 = 0; p.pad1 = 0;
+  p.dav = alpha_ub * p.sop * (1.f - p.sop);
   const bool valid = (t >= 0 && t < T);
   p.tmpl = valid ? t : -1;
+  p.tbase = valid ? t * Ht * Wt : 0;
+  if (!valid) {   // never drawn: an empty push box rejects every pixel
+    p.hw = -1.f;
+    p.hh = -1.f;
+  }
   recs[n] = p;
 
   int cnt = 0;
@@ -282,27 +287,35 @@ __global__ __launch_bounds__(256) void k_radix_rowscan(int* __restrict__ hist, i
 
 // Stable scatter.  Keys of block b are visited as wave w, item i, lane l
 // (index b*4096 + w*1024 + i*64 + l), i.e. in input order; each wave ranks its
-// items by digit with ballots and a per-wave running count in LDS.
+// items by digit with ballots and a per-wave running count in LDS.  The block
+// then reorders its keys by digit in LDS and writes each digit's run
+// contiguously, so global writes are coalesced.
 __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     const int* __restrict__ kin, const int* __restrict__ vin, int* __restrict__ kout,
     int* __restrict__ vout, const long long* __restrict__ total, long long cap, int shift,
     int bits, int nblk, const int* __restrict__ hist, const int* __restrict__ totals) {
-  __shared__ int dbase[256];
+  __shared__ int gbase[256];     // global position of this block's first key of digit d
+  __shared__ int loff[256];      // block-local offset of digit d
   __shared__ int wcnt[4][256];
+  __shared__ int skey[kRadixTile];
+  __shared__ int sval[kRadixTile];
   __shared__ long long sh[4];
   const int radix = 1 << bits, mask = radix - 1;
   const long long P = pairs_in_range(total, cap);
   const int w = threadIdx.x >> 6, l = lane_id();
+  const long long blk0 = (long long)blockIdx.x * kRadixTile;
+  if (blk0 >= P) return;   // uniform: nothing of this block is in range
+  const int nvalid = (int)min((long long)kRadixTile, P - blk0);
   {
     const int d = threadIdx.x;
     const long long v = d < radix ? totals[d] : 0;
     long long tot;
     const long long ex = block256_excl_scan(v, tot, sh);
-    if (d < radix) dbase[d] = (int)ex;
+    if (d < radix) gbase[d] = (int)ex + hist[(long)d * nblk + blockIdx.x];
   }
   for (int d = l; d < radix; d += 64) wcnt[w][d] = 0;
   __syncthreads();
-  const long long base = (long long)blockIdx.x * kRadixTile + w * (64 * kRadixItems) + l;
+  const long long base = blk0 + w * (64 * kRadixItems) + l;
   const unsigned long long lt = (1ull << l) - 1ull;
   int kk[kRadixItems], vv[kRadixItems], rk[kRadixItems];
 #pragma unroll
@@ -332,6 +345,8 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     rk[i] = r;
   }
   __syncthreads();
+  // exclusive prefix over waves per digit; block count per digit
+  long long cnt_d = 0;
   if (threadIdx.x < radix) {
     const int d = threadIdx.x;
     int s = 0;
@@ -341,33 +356,51 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
       wcnt[q][d] = s;
       s += t;
     }
+    cnt_d = s;
+  }
+  {
+    long long tot;
+    const long long ex = block256_excl_scan(cnt_d, tot, sh);
+    if (threadIdx.x < radix) loff[threadIdx.x] = (int)ex;
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kRadixItems; ++i) {
     if (base + i * 64 < P) {
       const int d = (kk[i] >> shift) & mask;
-      const long long pos = (long long)dbase[d] + hist[(long)d * nblk + blockIdx.x] + wcnt[w][d] + rk[i];
-      kout[pos] = kk[i];
-      vout[pos] = vv[i];
+      const int lp = loff[d] + wcnt[w][d] + rk[i];
+      skey[lp] = kk[i];
+      sval[lp] = vv[i];
     }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nvalid; i += kRadixThreads) {
+    const int k = skey[i];
+    const int d = (k >> shift) & mask;
+    const long long pos = (long long)gbase[d] + (i - loff[d]);
+    kout[pos] = k;
+    vout[pos] = sval[i];
   }
 }
 
-// tile_offsets[t] = first index whose tile id >= t; tile_offsets[tiles] = P.
-__global__ __launch_bounds__(256) void k_tile_ranges(const int* __restrict__ keys,
+// tile_offsets[t] = first index whose tile id >= t, tile_offsets[tiles] = P,
+// from the boundaries of the sorted ids.  On overflow (P > cap) every offset
+// is 0.  Needs max(cap, tiles) + 1 threads.
+__global__ __launch_bounds__(256) void k_tile_bounds(const int* __restrict__ keys,
                                                      const long long* __restrict__ total,
                                                      long long cap, int tiles,
                                                      int* __restrict__ tile_offsets) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > tiles) return;
-  const long long P = pairs_in_range(total, cap);
-  long long lo = 0, hi = P;
-  while (lo < hi) {
-    const long long mid = (lo + hi) >> 1;
-    if (keys[mid] < t) lo = mid + 1; else hi = mid;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long Praw = *total;
+  if (Praw > cap) {
+    if (i <= tiles) tile_offsets[i] = 0;
+    return;
   }
-  tile_offsets[t] = (int)lo;
+  const long long P = Praw;
+  if (i > P) return;
+  const int kcur = (i < P) ? keys[i] : tiles;
+  const int kprev = (i > 0) ? keys[i - 1] : -1;
+  for (int t = kprev + 1; t <= kcur; ++t) tile_offsets[t] = (int)i;
 }
 
 }  // namespace dbmp

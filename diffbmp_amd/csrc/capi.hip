@@ -190,6 +190,8 @@ int check_bitmaps(const dbmp_bitmaps* b) {
     return fail(DBMP_E_ARG, "templates must be (T>0, H>=2, W>=2), got (%d,%d,%d)",
                 b->num_templates, b->height, b->width);
   if (b->dtype < DBMP_F32 || b->dtype > DBMP_BF16) return fail(DBMP_E_ARG, "bad bitmap dtype %d", b->dtype);
+  if ((long long)b->num_templates * b->height * b->width >= (1LL << 31))
+    return fail(DBMP_E_ARG, "templates hold more than 2^31 texels");
   if (b->colors_orig_mode < 0 || b->colors_orig_mode > 2)
     return fail(DBMP_E_ARG, "bad colors_orig_mode %d", b->colors_orig_mode);
   if (b->colors_orig_mode != 0 && !b->colors_orig) return fail(DBMP_E_ARG, "colors_orig is NULL");
@@ -254,6 +256,8 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.recs = h->recs;
   A.toffs = L->tile_offsets;
   A.tidx = L->tile_indices;
+  A.pcount = (const long long*)L->pair_count;
+  A.pcap = L->capacity;
   A.quads = h->quads;
   const bool blend = b->colors_orig_mode != 0 && b->c_blend > 0.f;
   A.cmap = blend ? b->colors_orig : nullptr;
@@ -262,6 +266,7 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.N = p->num_primitives;
   A.Ht = b->height;
   A.Wt = b->width;
+  A.qmax = b->num_templates * b->height * b->width - 1;
   A.Wtm1 = (float)(b->width - 1);
   A.Htm1 = (float)(b->height - 1);
   A.hsx = 0.5f * (float)(b->width - 1);
@@ -431,9 +436,9 @@ int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int 
       std::swap(vcur, valt);
     }
   }
-  hipLaunchKernelGGL(k_tile_ranges, dim3(ceil_div(h->tiles + 1, 256)), dim3(256), 0, st, kcur,
-                     dtot, (long long)capacity, h->tiles, tile_offsets);
-  LAUNCH_CHECK("k_tile_ranges");
+  hipLaunchKernelGGL(k_tile_bounds, dim3(ceil_div(std::max<long long>(capacity, h->tiles) + 1, 256)),
+                     dim3(256), 0, st, kcur, dtot, (long long)capacity, h->tiles, tile_offsets);
+  LAUNCH_CHECK("k_tile_bounds");
   return 0;
 }
 
@@ -470,7 +475,7 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
   const RasterArgs A = make_args(h, prims, bm, lists, eps_T);
   const bool blend = A.cmap != nullptr;
-  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(h->tiles * h->subs), dim3(64), 0, st, A,
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(h->tiles * h->subs), dim3(kThreads), 0, st, A,
                     out_color, out_alpha, out_T_final, out_last);
   LAUNCH_CHECK("k_raster_fwd");
   timing_end(h, h->ev_fwd, st);
@@ -503,7 +508,7 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   }
   const RasterArgs A = make_args(h, prims, bm, lists, 0.f);
   const bool blend = A.cmap != nullptr;
-  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd, dim3(h->tiles * h->subs), dim3(64), 0, st, A,
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd, dim3(h->tiles * h->subs), dim3(kThreads), 0, st, A,
                     grad_color, grad_alpha, out_color, out_T_final, out_last, h->grad8);
   LAUNCH_CHECK("k_raster_bwd");
   if (N > 0) {

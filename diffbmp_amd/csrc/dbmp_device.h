@@ -22,7 +22,8 @@ struct alignas(16) PrimRec {
   float cr, cg, cb, sop;     // sigmoid(colour logits), sigmoid(opacity logit)
   int tmpl;                  // template index, -1 => never pushed
   float r;                   // radius
-  int pad0, pad1;
+  int tbase;                 // tmpl * Ht * Wt: first texel quad of the template
+  float dav;                 // alpha_ub * s (1 - s): d alpha / d(opacity logit) per unit mask
 };
 static_assert(sizeof(PrimRec) == 64, "PrimRec is 64 bytes");
 
@@ -80,7 +81,7 @@ __device__ __forceinline__ float sigmoid_safe(float x) {
 // Per-sample geometry in exact fp32 (tile_forward.cu:187-267).  Returns
 // false when the pixel is outside the push box or the texture coordinates
 // fall outside [0, W-1] x [0, H-1] (bilinear_sample returns 0 there, and a
-// zero mask is never pushed).
+// zero mask is never pushed), or are NaN (undefined in the reference).
 struct Geo {
   float u, v, ndx, ndy, tx, ty, wx, wy;
   int x0, y0;
@@ -98,10 +99,11 @@ __device__ __forceinline__ bool sample_geom(const PrimRec& p, float px, float py
   // scaling is exact on both sides.
   const float tx = (u + 1.f) * hsx;
   const float ty = (v + 1.f) * hsy;
-  if (tx < 0.f || tx > Wtm1 || ty < 0.f || ty > Htm1) return false;
+  // written so that NaN coordinates are rejected as well
+  if (!(tx >= 0.f && tx <= Wtm1 && ty >= 0.f && ty <= Htm1)) return false;
   const float fx = floorf(tx), fy = floorf(ty);
-  g.x0 = min(max((int)fx, 0), Wt - 1);
-  g.y0 = min(max((int)fy, 0), Ht - 1);
+  g.x0 = (int)fx;   // in [0, Wt-1] after the test above
+  g.y0 = (int)fy;
   g.wx = tx - fx;
   g.wy = ty - fy;
   g.u = u; g.v = v; g.ndx = ndx; g.ndy = ndy; g.tx = tx; g.ty = ty;
@@ -157,32 +159,48 @@ __device__ __forceinline__ int wave_max_i32(int v) {
   return v;
 }
 
-// Reduce-scatter of eight per-lane values: afterwards every lane of the
-// 8-lane group g = lane >> 3 holds the wave total of value index
-// ((lane>>5)&1)*4 + ((lane>>4)&1)*2 + ((lane>>3)&1).  10 shuffles instead of
-// 48 for eight independent butterflies.
+// DPP lane moves within 16-lane rows (gfx9 encodings).
+constexpr int kDppQuadXor1 = 0xB1;     // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;     // quad_perm [2,3,0,1]
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowRor8 = 0x128;     // row_ror:8 == lane ^ 8 inside a row
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+
+// Reduce-scatter of eight per-lane values across the wave: afterwards every
+// lane of the 8-lane group g = lane >> 3 holds the wave total of value index
+// ((lane>>5)&1)*4 + ((lane>>4)&1)*2 + ((lane>>3)&1).  Two halving steps use
+// the gfx950 lane-swap instructions (v_permlane32_swap exchanges the upper
+// half of one register with the lower half of another, v_permlane16_swap the
+// odd rows of one with the even rows of another), so each pair of values
+// costs one swap and one add and no LDS traffic; the rest are DPP adds.
 __device__ __forceinline__ float wave_reduce_scatter8(const float v[8], int& which) {
   const int l = lane_id();
-  const bool b5 = (l >> 5) & 1, b4 = (l >> 4) & 1, b3 = (l >> 3) & 1;
   float w[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const float mine = b5 ? v[4 + i] : v[i];
-    const float other = b5 ? v[i] : v[4 + i];
-    w[i] = mine + __shfl_xor(other, 32, 64);
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]), __float_as_uint(v[4 + i]),
+                                                    false, false);
+    w[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
   }
   float x[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const float mine = b4 ? w[2 + i] : w[i];
-    const float other = b4 ? w[i] : w[2 + i];
-    x[i] = mine + __shfl_xor(other, 16, 64);
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(w[i]), __float_as_uint(w[2 + i]),
+                                                    false, false);
+    x[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
   }
-  float y = (b3 ? x[1] : x[0]) + __shfl_xor(b3 ? x[0] : x[1], 8, 64);
-  y += __shfl_xor(y, 4, 64);
-  y += __shfl_xor(y, 2, 64);
-  y += __shfl_xor(y, 1, 64);
-  which = (b5 ? 4 : 0) + (b4 ? 2 : 0) + (b3 ? 1 : 0);
+  const bool b3 = (l >> 3) & 1;
+  const float s0 = x[0] + dpp<kDppRowRor8>(x[0]);
+  const float s1 = x[1] + dpp<kDppRowRor8>(x[1]);
+  float y = b3 ? s1 : s0;
+  y += dpp<kDppQuadXor1>(y);
+  y += dpp<kDppQuadXor2>(y);
+  y += dpp<kDppRowHalfMirror>(y);
+  which = (((l >> 5) & 1) << 2) | (((l >> 4) & 1) << 1) | (b3 ? 1 : 0);
   return y;
 }
 

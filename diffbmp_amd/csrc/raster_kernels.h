@@ -1,12 +1,14 @@
 // Raster forward / backward over tile lists.
 //
-// Work decomposition: one wave64 owns a 16x16 pixel block ("sub-tile") and
-// walks the tile list of the binning tile that contains it; each lane owns the
-// 4 pixels (x0 + lane%16, y0 + lane/16 + 4j), j = 0..3.  The wave's loop over
-// list entries is uniform, so each entry's primitive id and 64-byte PrimRec
-// come in through scalar loads and every per-sample operand that depends only
-// on the primitive sits in SGPRs.  No LDS is needed for the parameter
-// batch and no barrier is ever taken: waves are independent.
+// Work decomposition: one 256-thread workgroup (4 wave64s) owns a 16x16 pixel
+// block ("sub-tile") and walks the tile list of the binning tile containing
+// it; wave w owns pixel rows 4w..4w+3, one pixel per lane.  The list is
+// consumed in batches of 64 entries whose 64-byte primitive records are
+// staged in LDS, double buffered: while the waves composite batch b, the
+// records of batch b+1 are already in flight from L2 into registers.  Each
+// wave culls a list entry against its own 16x4 pixel rectangle and stops as
+// soon as every lane's transmittance is below eps_T; the workgroup stops when
+// all four waves have.
 //
 // Semantics follow the reference kernels (tile_forward.cu:96-398,
 // tile_backward.cu:101-452) with two documented changes:
@@ -28,11 +30,14 @@ struct RasterArgs {
   const PrimRec* recs;
   const int* toffs;
   const int* tidx;
+  const long long* pcount;  // binning pair count (device) or null
+  long long pcap;           // capacity the lists were built with
   const void* quads;
   const void* cmap;      // colour maps (M,Ht,Wt,3) or null
   int cmap_per_prim;     // 1: M = N (indexed by primitive), 0: M = T (by template)
   float c_blend;
   int N, Ht, Wt;
+  int qmax;              // last valid texel-quad index (T*Ht*Wt - 1)
   float Wtm1, Htm1, hsx, hsy;
   int H, W, ts, tilesX, tiles, subs;   // subs = (ts/16)^2 sub-tiles per tile
   float eps;
@@ -64,202 +69,305 @@ __device__ __forceinline__ void sample_color(const RasterArgs& A, const PrimRec&
   }
 }
 
-template <int DT, bool BLEND>
-__global__ __launch_bounds__(64) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
-                                                   float* __restrict__ out_alpha,
-                                                   float* __restrict__ out_T,
-                                                   int* __restrict__ out_last) {
-  using Q = QuadT<DT>;
+constexpr int kBatch = 64;    // list entries staged per batch
+constexpr int kThreads = 256; // 4 waves per 16x16 block
+
+struct BlockGeom {
+  int tile, bx, by, px, py, w, lane;
+};
+
+__device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A) {
+  BlockGeom g;
   const int b = xcd_remap(blockIdx.x, A.tiles * A.subs);
-  const int tile = b / A.subs, sub = b - tile * A.subs;
+  g.tile = b / A.subs;
+  const int sub = b - g.tile * A.subs;
   const int spr = A.ts / kSub;
-  const int tx = tile % A.tilesX, ty = tile / A.tilesX;
-  const int bx = tx * A.ts + (sub % spr) * kSub, by = ty * A.ts + (sub / spr) * kSub;
-  const int lane = lane_id();
-  const int px = bx + (lane & 15);
-  const int py0 = by + (lane >> 4);
-  const float fpx = (float)px;
+  const int tx = g.tile % A.tilesX, ty = g.tile / A.tilesX;
+  g.bx = tx * A.ts + (sub % spr) * kSub;
+  g.by = ty * A.ts + (sub / spr) * kSub;
+  g.w = threadIdx.x >> 6;
+  g.lane = threadIdx.x & 63;
+  g.px = g.bx + (g.lane & 15);
+  g.py = g.by + 4 * g.w + (g.lane >> 4);
+  return g;
+}
 
-  float T[4], C[4][3];
-  int last[4];
-  bool alive[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    alive[j] = (px < A.W) && (py0 + 4 * j < A.H);
-    T[j] = 1.f;
-    C[j][0] = C[j][1] = C[j][2] = 0.f;
-    last[j] = -1;
+// List range of a tile.  If binning overflowed its capacity the lists were
+// not written: walk all primitives in index order (same result, slower).
+__device__ __forceinline__ bool list_range(const RasterArgs& A, int tile, int& start, int& end) {
+  if (A.pcount != nullptr && *A.pcount > A.pcap) {
+    start = 0;
+    end = A.N;
+    return true;
   }
-  const int start = A.toffs[tile], end = A.toffs[tile + 1];
-  // conservative (1 px slack) block rectangle for the per-entry cull
-  const float rx0 = (float)bx - 1.f, rx1 = (float)(bx + kSub), ry0 = (float)by - 1.f,
-              ry1 = (float)(by + kSub);
-  const typename Q::type* quads = (const typename Q::type*)A.quads;
+  start = A.toffs[tile];
+  end = A.toffs[tile + 1];
+  return false;
+}
 
-  for (int k = start; k < end; ++k) {
-    if (!__any(alive[0] | alive[1] | alive[2] | alive[3])) break;
-    const int n = A.tidx[k];
-    if ((unsigned)n >= (unsigned)A.N) continue;
-    const PrimRec p = A.recs[n];
-    if (p.tmpl < 0) continue;
-    if (p.mx - p.hw > rx1 || p.mx + p.hw < rx0 || p.my - p.hh > ry1 || p.my + p.hh < ry0) continue;
-    const long tb = (long)p.tmpl * A.Ht * A.Wt;
+// Load one 16-byte quarter of the record of list entry k (thread t handles
+// quarter t&3 of record t>>2); entries past the end get an empty push box.
+__device__ __forceinline__ float4 fetch_rec_part(const RasterArgs& A, bool all, int k, int end,
+                                                 int part, int& n_out) {
+  const int n = (k < end) ? (all ? k : A.tidx[k]) : -1;
+  const bool ok = (unsigned)n < (unsigned)A.N;
+  n_out = ok ? n : -1;
+  if (ok) return reinterpret_cast<const float4*>(A.recs + n)[part];
+  return part == 0 ? make_float4(0.f, 0.f, -1.f, -1.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Bit w set when the push box (quarter 0 of the record: mx, my, hw, hh) can
+// touch wave w's pixels [bx, bx+15] x [by+4w, by+4w+3]; 1 px slack keeps the
+// test conservative w.r.t. the per-pixel fp32 test.
+__device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
+  const float mx = q.x, my = q.y, hw = q.z, hh = q.w;
+  if (!(hw >= 0.f && hh >= 0.f)) return 0;
+  if (mx - hw > (float)(bx + 16) || mx + hw < (float)(bx - 1)) return 0;
+  int m = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (!alive[j]) continue;
-      Geo g;
-      if (!sample_geom(p, fpx, (float)(py0 + 4 * j), A.Wtm1, A.Htm1, A.hsx, A.hsy, A.Wt, A.Ht, g))
-        continue;
-      float v00, v01, v10, v11;
-      Q::get(quads, tb + (long)g.y0 * A.Wt + g.x0, v00, v01, v10, v11);
-      float m = bilerp(v00, v01, v10, v11, g.wx, g.wy);
-      m = fmaxf(0.f, fminf(1.f, m));
-      if (m <= 1e-6f) continue;
-      const float a = p.abase * m;
-      float col[3];
-      sample_color<DT, BLEND>(A, p, n, g, col);
-      const float w = T[j] * a;
-      C[j][0] = fmaf(w, col[0], C[j][0]);
-      C[j][1] = fmaf(w, col[1], C[j][1]);
-      C[j][2] = fmaf(w, col[2], C[j][2]);
-      T[j] = T[j] * fmaxf(1.f - a, 0.f);
-      if (T[j] < A.eps) {
-        alive[j] = false;
-        last[j] = k - start + 1;
+  for (int w = 0; w < 4; ++w) {
+    const float y0 = (float)(by + 4 * w);
+    if (!(my - hh > y0 + 4.f || my + hh < y0 - 1.f)) m |= 1 << w;
+  }
+  return m;
+}
+
+// Stage batch entry ri (one quarter per thread) into LDS slot `buf`.
+struct Stage {
+  float4 v;
+  int n;
+};
+__device__ __forceinline__ Stage stage_load(const RasterArgs& A, bool all, int k, int end, int part) {
+  Stage st;
+  st.v = fetch_rec_part(A, all, k, end, part, st.n);
+  return st;
+}
+__device__ __forceinline__ void stage_store(const Stage& st, PrimRec (*srec)[kBatch], int (*sid)[kBatch],
+                                            int (*smask)[kBatch], int buf, int ri, int part, int bx,
+                                            int by) {
+  reinterpret_cast<float4*>(&srec[buf][ri])[part] = st.v;
+  if (part == 0) {
+    sid[buf][ri] = st.n;
+    smask[buf][ri] = wave_mask(st.v, bx, by);
+  }
+}
+
+// One sample, branch-free: geometry with the exact fp32 decisions of
+// sample_geom, one quad gather (a rejected lane gathers quad 0), bilinear
+// mask.  `pushed` is the reference's push condition (box, texture bounds,
+// mask > 1e-6) and the caller's activity flag.
+template <int DT>
+struct Sample {
+  float m, gxt, gyt, u, v, ndx, ndy, tx, ty;
+  bool pushed;
+};
+
+template <int DT, bool GRAD>
+__device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const PrimRec& p, float fpx,
+                                                  float fpy, bool active) {
+  using Q = QuadT<DT>;
+  Sample<DT> r;
+  const float dx = fpx - p.mx, dy = fpy - p.my;
+  const bool inbox = fabsf(dx) <= p.hw && fabsf(dy) <= p.hh;
+  r.ndx = dx * p.rinv;
+  r.ndy = dy * p.rinv;
+  r.u = p.c * r.ndx + p.s * r.ndy;
+  r.v = -p.s * r.ndx + p.c * r.ndy;
+  r.tx = (r.u + 1.f) * A.hsx;
+  r.ty = (r.v + 1.f) * A.hsy;
+  const bool inb = r.tx >= 0.f && r.tx <= A.Wtm1 && r.ty >= 0.f && r.ty <= A.Htm1;
+  const bool ok = active && inbox && inb;
+  const float fx = floorf(r.tx), fy = floorf(r.ty);
+  // clamped rather than selected, so the gather is unconditional; a lane
+  // that is not `ok` reads some valid quad and its sample is discarded
+  const int idx = min(max(p.tbase + (int)fy * A.Wt + (int)fx, 0), A.qmax);
+  float v00, v01, v10, v11;
+  Q::get((const typename Q::type*)A.quads, idx, v00, v01, v10, v11);
+  const float wx = r.tx - fx, wy = r.ty - fy;
+  float m;
+  if constexpr (GRAD) {
+    m = bilerp_grad(v00, v01, v10, v11, wx, wy, r.gxt, r.gyt);
+  } else {
+    m = bilerp(v00, v01, v10, v11, wx, wy);
+    r.gxt = r.gyt = 0.f;
+  }
+  r.m = fmaxf(0.f, fminf(1.f, m));
+  r.pushed = ok && r.m > 1e-6f;
+  return r;
+}
+
+template <int DT, bool BLEND>
+__global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
+                                                         float* __restrict__ out_alpha,
+                                                         float* __restrict__ out_T,
+                                                         int* __restrict__ out_last) {
+  __shared__ PrimRec srec[2][kBatch];
+  __shared__ int sid[2][kBatch];
+  __shared__ int smask[2][kBatch];
+  const BlockGeom G = block_geom(A);
+  const float fpx = (float)G.px, fpy = (float)G.py;
+  bool alive = (G.px < A.W) && (G.py < A.H);
+  float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+  int last = -1;
+  int start, end;
+  const bool all = list_range(A, G.tile, start, end);
+  const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
+  const int wbit = 1 << G.w;
+  stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
+  __syncthreads();
+  int buf = 0;
+  for (int base = start; base < end; base += kBatch, buf ^= 1) {
+    const bool have_next = base + kBatch < end;
+    Stage nxt;
+    if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
+    const int cnt = min(kBatch, end - base);
+    if (__any(alive)) {
+      for (int j = 0; j < cnt; ++j) {
+        if (!(__builtin_amdgcn_readfirstlane(smask[buf][j]) & wbit)) continue;
+        const PrimRec p = srec[buf][j];
+        const Sample<DT> S = eval_sample<DT, false>(A, p, fpx, fpy, alive);
+        float col[3];
+        Geo geo;
+        geo.tx = S.tx;
+        geo.ty = S.ty;
+        sample_color<DT, BLEND>(A, p, sid[buf][j], geo, col);
+        const float a = S.pushed ? p.abase * S.m : 0.f;   // a = 0: compositing is a no-op
+        const float wgt = T * a;
+        C0 = fmaf(wgt, col[0], C0);
+        C1 = fmaf(wgt, col[1], C1);
+        C2 = fmaf(wgt, col[2], C2);
+        T = T * fmaxf(1.f - a, 0.f);
+        const bool dead = S.pushed && T < A.eps;
+        last = dead ? base - start + j + 1 : last;
+        alive = alive && !dead;
+        if (!__any(alive)) break;
       }
     }
+    if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G.bx, G.by);
+    if (!__syncthreads_or(alive)) break;
   }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int py = py0 + 4 * j;
-    if (px < A.W && py < A.H) {
-      const long p = (long)py * A.W + px;
-      out_color[3 * p + 0] = C[j][0];
-      out_color[3 * p + 1] = C[j][1];
-      out_color[3 * p + 2] = C[j][2];
-      out_alpha[p] = 1.f - T[j];
-      out_T[p] = T[j];
-      out_last[p] = last[j] < 0 ? end - start : last[j];
-    }
+  if (G.px < A.W && G.py < A.H) {
+    const long p = (long)G.py * A.W + G.px;
+    out_color[3 * p + 0] = C0;
+    out_color[3 * p + 1] = C1;
+    out_color[3 * p + 2] = C2;
+    out_alpha[p] = 1.f - T;
+    out_T[p] = T;
+    out_last[p] = last < 0 ? end - start : last;
   }
 }
 
 // grad8[n*8 + i]: 0 mx, 1 my, 2 r, 3 theta, 4 opacity logit, 5..7 colour logits
 template <int DT, bool BLEND>
-__global__ __launch_bounds__(64) void k_raster_bwd(RasterArgs A, const float* __restrict__ gCin,
-                                                   const float* __restrict__ gAin,
-                                                   const float* __restrict__ Cfin,
-                                                   const float* __restrict__ Tfin,
-                                                   const int* __restrict__ lastin,
-                                                   float* __restrict__ grad8) {
-  using Q = QuadT<DT>;
-  const int b = xcd_remap(blockIdx.x, A.tiles * A.subs);
-  const int tile = b / A.subs, sub = b - tile * A.subs;
-  const int spr = A.ts / kSub;
-  const int tx = tile % A.tilesX, ty = tile / A.tilesX;
-  const int bx = tx * A.ts + (sub % spr) * kSub, by = ty * A.ts + (sub / spr) * kSub;
-  const int lane = lane_id();
-  const int px = bx + (lane & 15);
-  const int py0 = by + (lane >> 4);
-  const float fpx = (float)px;
-
-  float T[4], gc[4][3], ga[4], cf[4][3], cacc[4][3], tf[4];
-  int lst[4];
-  int kmax = 0;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int py = py0 + 4 * j;
-    const bool ok = (px < A.W) && (py < A.H);
-    const long p = ok ? (long)py * A.W + px : 0;
-    T[j] = 1.f;
-    lst[j] = ok ? lastin[p] : 0;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      gc[j][ch] = ok ? gCin[3 * p + ch] : 0.f;
-      cf[j][ch] = ok ? Cfin[3 * p + ch] : 0.f;
-      cacc[j][ch] = 0.f;
-    }
-    ga[j] = (ok && gAin) ? gAin[p] : 0.f;
-    tf[j] = ok ? Tfin[p] : 0.f;
-    kmax = max(kmax, lst[j]);
-  }
-  const int start = A.toffs[tile];
-  kmax = min(wave_max_i32(kmax), A.toffs[tile + 1] - start);
-  const float rx0 = (float)bx - 1.f, rx1 = (float)(bx + kSub), ry0 = (float)by - 1.f,
-              ry1 = (float)(by + kSub);
-  const typename Q::type* quads = (const typename Q::type*)A.quads;
-
-  for (int kk = 0; kk < kmax; ++kk) {
-    const int n = A.tidx[start + kk];
-    if ((unsigned)n >= (unsigned)A.N) continue;
-    const PrimRec p = A.recs[n];
-    if (p.tmpl < 0) continue;
-    if (p.mx - p.hw > rx1 || p.mx + p.hw < rx0 || p.my - p.hh > ry1 || p.my + p.hh < ry0) continue;
-    const long tb = (long)p.tmpl * A.Ht * A.Wt;
-    const float dsc0 = p.cr * (1.f - p.cr), dsc1 = p.cg * (1.f - p.cg), dsc2 = p.cb * (1.f - p.cb);
-    const float dav = p.abase * (1.f - p.sop);   // d alpha / d v = alpha_ub s (1-s) per unit mask
-    float acc[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) acc[i] = 0.f;
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (kk >= lst[j]) continue;
-      Geo g;
-      if (!sample_geom(p, fpx, (float)(py0 + 4 * j), A.Wtm1, A.Htm1, A.hsx, A.hsy, A.Wt, A.Ht, g))
-        continue;
-      float v00, v01, v10, v11, gxt, gyt;
-      Q::get(quads, tb + (long)g.y0 * A.Wt + g.x0, v00, v01, v10, v11);
-      float m = bilerp_grad(v00, v01, v10, v11, g.wx, g.wy, gxt, gyt);
-      m = fmaxf(0.f, fminf(1.f, m));
-      if (m <= 1e-6f) continue;
-      any = true;
-      const float a = p.abase * m;
+__global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const float* __restrict__ gCin,
+                                                         const float* __restrict__ gAin,
+                                                         const float* __restrict__ Cfin,
+                                                         const float* __restrict__ Tfin,
+                                                         const int* __restrict__ lastin,
+                                                         float* __restrict__ grad8) {
+  __shared__ PrimRec srec[2][kBatch];
+  __shared__ int sid[2][kBatch];
+  __shared__ int smask[2][kBatch];
+  __shared__ float sred[4][kBatch][8];
+  __shared__ int skmax;
+  const BlockGeom G = block_geom(A);
+  const float fpx = (float)G.px, fpy = (float)G.py;
+  const bool okpix = (G.px < A.W) && (G.py < A.H);
+  const long pix = okpix ? (long)G.py * A.W + G.px : 0;
+  const int lst = okpix ? lastin[pix] : 0;
+  const float gc0 = okpix ? gCin[3 * pix] : 0.f, gc1 = okpix ? gCin[3 * pix + 1] : 0.f,
+              gc2 = okpix ? gCin[3 * pix + 2] : 0.f;
+  const float ga = (okpix && gAin) ? gAin[pix] : 0.f;
+  const float cf0 = okpix ? Cfin[3 * pix] : 0.f, cf1 = okpix ? Cfin[3 * pix + 1] : 0.f,
+              cf2 = okpix ? Cfin[3 * pix + 2] : 0.f;
+  const float tf = okpix ? Tfin[pix] : 0.f;
+  float T = 1.f, ca0 = 0.f, ca1 = 0.f, ca2 = 0.f;
+  int start, end;
+  const bool all = list_range(A, G.tile, start, end);
+  const int wkmax = wave_max_i32(lst);
+  if (threadIdx.x == 0) skmax = 0;
+  for (int i = threadIdx.x; i < 4 * kBatch * 8; i += kThreads) (&sred[0][0][0])[i] = 0.f;
+  __syncthreads();
+  if (G.lane == 0) atomicMax(&skmax, wkmax);
+  const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
+  const int wbit = 1 << G.w;
+  stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
+  __syncthreads();
+  const int stop = start + min(skmax, end - start);
+  int buf = 0;
+  for (int base = start; base < stop; base += kBatch, buf ^= 1) {
+    const bool have_next = base + kBatch < stop;
+    Stage nxt;
+    if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
+    const int cnt = min(kBatch, stop - base);
+    for (int j = 0; j < cnt; ++j) {
+      const int kk = base - start + j;
+      if (kk >= wkmax) break;
+      if (!(__builtin_amdgcn_readfirstlane(smask[buf][j]) & wbit)) continue;
+      const PrimRec p = srec[buf][j];
+      const Sample<DT> S = eval_sample<DT, true>(A, p, fpx, fpy, kk < lst);
+      if (!__any(S.pushed)) continue;
       float col[3];
-      sample_color<DT, BLEND>(A, p, n, g, col);
-      const float Tk = T[j];
-      const float w = Tk * a;
-      // suffix colour S_k = C_final - sum_{m<=k} T_m a_m c_m; the prefix is
-      // recomputed with the forward's exact operations, so S is exactly 0
-      // after the last composited layer
-      float S[3];
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        cacc[j][ch] = fmaf(w, col[ch], cacc[j][ch]);
-        S[ch] = cf[j][ch] - cacc[j][ch];
-      }
+      Geo geo;
+      geo.tx = S.tx;
+      geo.ty = S.ty;
+      sample_color<DT, BLEND>(A, p, sid[buf][j], geo, col);
+      const float a = S.pushed ? p.abase * S.m : 0.f;
+      const float Tk = T;
+      const float wgt = Tk * a;
+      // suffix colour S_k = C_final - sum_{m<=k} T_m a_m c_m, the prefix
+      // recomputed with the forward's exact operations (S = 0 at the end)
+      ca0 = fmaf(wgt, col[0], ca0);
+      ca1 = fmaf(wgt, col[1], ca1);
+      ca2 = fmaf(wgt, col[2], ca2);
+      const float S0 = cf0 - ca0, S1 = cf1 - ca1, S2 = cf2 - ca2;
       const float om = 1.f - a;
       const float inv1m = __builtin_amdgcn_rcpf(fmaxf(om, 1e-6f));
       const float Tn = Tk * fmaxf(om, 0.f);
       // suffix transmittance B_k = T_final / T_{k+1}; 1 if nothing follows
-      const float B = Tn > 0.f ? fminf(tf[j] * __builtin_amdgcn_rcpf(Tn), 1.f) : 1.f;
-      float dLda = ga[j] * (Tk * B);
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) dLda = fmaf(gc[j][ch], fmaf(Tk, col[ch], -S[ch] * inv1m), dLda);
-      // colour logits (tile_backward.cu:255-261)
-      acc[5] = fmaf(gc[j][0] * w, dsc0, acc[5]);
-      acc[6] = fmaf(gc[j][1] * w, dsc1, acc[6]);
-      acc[7] = fmaf(gc[j][2] * w, dsc2, acc[7]);
-      // opacity logit (:311-312)
-      acc[4] = fmaf(dLda, m * dav, acc[4]);
+      const float B = Tn > 0.f ? fminf(tf * __builtin_amdgcn_rcpf(Tn), 1.f) : 1.f;
+      float dLda = ga * (Tk * B);
+      dLda = fmaf(gc0, fmaf(Tk, col[0], -S0 * inv1m), dLda);
+      dLda = fmaf(gc1, fmaf(Tk, col[1], -S1 * inv1m), dLda);
+      dLda = fmaf(gc2, fmaf(Tk, col[2], -S2 * inv1m), dLda);
+      float acc[8];
+      // colour logits (tile_backward.cu:255-261); wgt = 0 for lanes not pushed
+      acc[5] = gc0 * wgt * (p.cr * (1.f - p.cr));
+      acc[6] = gc1 * wgt * (p.cg * (1.f - p.cg));
+      acc[7] = gc2 * wgt * (p.cb * (1.f - p.cb));
+      // opacity logit (:311-312): alpha_ub * mask * s (1 - s)
+      const float me = S.pushed ? S.m : 0.f;
+      acc[4] = dLda * (me * p.dav);
       // mask path (:314-343)
-      const float dLdm = dLda * p.abase;
-      const float dLdu = dLdm * (gxt * A.hsx);
-      const float dLdv = dLdm * (gyt * A.hsy);
+      const float dLdm = S.pushed ? dLda * p.abase : 0.f;
+      const float dLdu = dLdm * (S.gxt * A.hsx);
+      const float dLdv = dLdm * (S.gyt * A.hsy);
       const float pc = p.c * p.rinv, ps = p.s * p.rinv;
-      acc[0] = fmaf(dLdu, -pc, fmaf(dLdv, ps, acc[0]));
-      acc[1] = fmaf(dLdu, -ps, fmaf(dLdv, -pc, acc[1]));
-      acc[2] = fmaf(-(fmaf(dLdu, g.u, dLdv * g.v)), p.rinv, acc[2]);
-      const float dudp = fmaf(-p.s, g.ndx, p.c * g.ndy);
-      const float dvdp = fmaf(-p.c, g.ndx, -p.s * g.ndy);
-      acc[3] = fmaf(dLdu, dudp, fmaf(dLdv, dvdp, acc[3]));
-      T[j] = Tn;
-    }
-    if (__any(any)) {
+      acc[0] = fmaf(dLdu, -pc, dLdv * ps);
+      acc[1] = fmaf(dLdu, -ps, -dLdv * pc);
+      acc[2] = -(fmaf(dLdu, S.u, dLdv * S.v)) * p.rinv;
+      const float dudp = fmaf(-p.s, S.ndx, p.c * S.ndy);
+      const float dvdp = fmaf(-p.c, S.ndx, -p.s * S.ndy);
+      acc[3] = fmaf(dLdu, dudp, dLdv * dvdp);
+      T = Tn;
       int which;
       const float tot = wave_reduce_scatter8(acc, which);
-      if ((lane & 7) == 0) unsafeAtomicAdd(&grad8[(long)n * 8 + which], tot);
+      if ((G.lane & 7) == 0) sred[G.w][j][which] = tot;
     }
+    if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G.bx, G.by);
+    __syncthreads();
+    // one atomic per (tile, entry, value): sum the four waves' partials
+    for (int i = threadIdx.x; i < cnt * 8; i += kThreads) {
+      const int j = i >> 3, q = i & 7;
+      const float s = sred[0][j][q] + sred[1][j][q] + sred[2][j][q] + sred[3][j][q];
+      sred[0][j][q] = 0.f;
+      sred[1][j][q] = 0.f;
+      sred[2][j][q] = 0.f;
+      sred[3][j][q] = 0.f;
+      const int n = sid[buf][j];
+      if (s != 0.f && n >= 0) unsafeAtomicAdd(&grad8[(long)n * 8 + q], s);
+    }
+    __syncthreads();
   }
 }
 

@@ -114,14 +114,13 @@ class _Run:
 
     def __init__(self, handle: Handle, store_dtype, means2D, radii, rotations, opacities, colors,
                  colors_orig, templates, bmp_sel, c_blend, mapping, eps_T,
-                 colors_orig_per_template=False, tmpl_version=-1):
+                 colors_orig_per_template=False):
         for t, nm in ((means2D, "means2D"), (templates, "primitive_templates")):
             _native.require_gpu(t, nm)
         cv = lambda t: t.to(dtype=store_dtype).contiguous()
         self.means2D, self.radii, self.rotations = cv(means2D), cv(radii), cv(rotations)
         self.opacities, self.colors = cv(opacities), cv(colors)
-        self.templates = cv(templates)
-        self.tmpl_version = tmpl_version if self.templates.data_ptr() == templates.data_ptr() else -1
+        self.templates, self.ext, self.tmpl_version = handle.prepared_templates(templates, store_dtype)
         self.sel = bmp_sel.to(torch.int32).contiguous()
         if isinstance(c_blend, torch.Tensor):
             c_blend = float(c_blend.item())
@@ -137,10 +136,9 @@ class _Run:
         self.eps_T = float(eps_T)
         self.handle = handle
         if mapping is not None and mapping.numel() > 0:
-            self.offsets, self.indices = unpack_mapping(mapping, handle.tiles)
+            self.lists = _native.Lists(*unpack_mapping(mapping, handle.tiles))
         else:
-            ext = handle.template_extents(self.templates)
-            self.offsets, self.indices = handle.bin(self.prims(), ext, tuple(self.templates.shape))
+            self.lists = handle.bin_lazy(self.prims(), self.ext, tuple(self.templates.shape))
 
     def prims(self):
         return Handle.prims(self.means2D, self.radii, self.rotations, self.opacities, self.colors,
@@ -156,7 +154,7 @@ class _Run:
         out_alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
         out_T = torch.empty((H, W), dtype=torch.float32, device=dev)
         out_last = torch.empty((H, W), dtype=torch.int32, device=dev)
-        self.handle.forward(self.prims(), self.bitmaps(), self.offsets, self.indices, self.eps_T,
+        self.handle.forward(self.prims(), self.bitmaps(), self.lists, self.eps_T,
                             out_color, out_alpha, out_T, out_last)
         return out_color, out_alpha, out_T, out_last
 
@@ -170,7 +168,7 @@ class _Run:
         g_th = torch.empty(N, dtype=f32, device=dev)
         g_v = torch.empty(N, dtype=f32, device=dev)
         g_c = torch.empty((N, 3), dtype=f32, device=dev)
-        self.handle.backward(self.prims(), self.bitmaps(), self.offsets, self.indices, self.eps_T,
+        self.handle.backward(self.prims(), self.bitmaps(), self.lists, self.eps_T,
                              gC, gA, out_color, out_T, out_last, g_means, g_r, g_th, g_v, g_c)
         return g_means, g_r, g_th, g_v, g_c
 
@@ -186,7 +184,7 @@ def _apply_forward(ctx, kind, store_dtype, out_dtype, means2D, radii, rotations,
                          f"called with {image_height}x{image_width} tile {tile_size}")
     run = _Run(handle, store_dtype, means2D, radii, rotations, opacities, colors, colors_orig,
                primitive_templates, global_bmp_sel, c_blend, tile_primitive_mapping,
-               getattr(handle, "eps_T", DEFAULT_EPS_T), tmpl_version=primitive_templates._version)
+               getattr(handle, "eps_T", DEFAULT_EPS_T))
     out_color, out_alpha, out_T, out_last = run.forward()
     ctx.run = run
     ctx.save_for_backward(out_color, out_T, out_last)
@@ -263,8 +261,7 @@ class TileRasterizerBatchFunction(Function):
                 co = colors_orig[b] if colors_orig.dim() == 5 else colors_orig
             run = _Run(handle, store, means2D[b], radii[b], rotations[b], opacities[b], colors[b], co,
                        primitive_templates, global_bmp_sel, c_blend, mp,
-                       getattr(handle, "eps_T", DEFAULT_EPS_T),
-                       tmpl_version=primitive_templates._version)
+                       getattr(handle, "eps_T", DEFAULT_EPS_T))
             oc, oa, oT, ol = run.forward()
             runs.append(run)
             outs_c.append(oc)

@@ -64,10 +64,17 @@ typedef struct {
   int64_t version;          /* caller's content version of `templates`; -1 = unknown */
 } dbmp_bitmaps;
 
-/* CSR tile lists: tile_offsets (tiles+1), tile_indices (P). */
+/* CSR tile lists: tile_offsets (tiles+1), tile_indices (P).  When the lists
+ * come from dbmp_bin, pass its device pair count and the capacity it was
+ * given: if the count exceeded the capacity (the lists were not written),
+ * the raster kernels walk every primitive 0..N-1 for every tile instead, which
+ * gives the same image and gradients, only slower.  pair_count = NULL means
+ * the lists are complete (e.g. a caller-supplied mapping). */
 typedef struct {
   const int32_t* tile_offsets;
   const int32_t* tile_indices;
+  const int64_t* pair_count;
+  int64_t capacity;
 } dbmp_tile_lists;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -96,9 +103,11 @@ int dbmp_template_extents(dbmp_rasterizer* h, const dbmp_bitmaps* bm, float* ext
  * binning and packing (vector_renderer.py:194-252,401-444;
  * simple_tile_renderer.py:479-516,721-766).  `ext` is (T,4) from
  * dbmp_template_extents.  Writes the pair count P to *d_pair_count (device
- * int64).  If P > capacity nothing else is written except tile_offsets, which
- * is set to all zeros (empty lists); the caller grows its buffer and calls
- * again.  tile_indices must hold `capacity` int32. */
+ * int64).  If P > capacity the lists are not written (tile_offsets is set to
+ * zeros); passing d_pair_count and capacity on to dbmp_forward/backward in
+ * dbmp_tile_lists keeps results exact in that case, and the caller reads P
+ * whenever convenient to grow its buffer.  tile_indices must hold `capacity`
+ * int32.  No host synchronisation. */
 int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
              int tmpl_h, int tmpl_w, int32_t* tile_offsets, int32_t* tile_indices,
              int64_t capacity, int64_t* d_pair_count, void* stream);

@@ -145,7 +145,8 @@ def lists_from_dense(mask: np.ndarray):
 
 
 def bin_lists_np(means2D, radii, rotations, sel, ext, H, W, ts):
-    """Tile lists via per-primitive tile ranges (no dense mask)."""
+    """Tile lists via per-primitive tile ranges (no dense mask).  A primitive
+    whose template index is out of range is never drawn and is in no list."""
     bb = bin_bboxes_np(means2D, radii, rotations, sel, ext)
     tX, tY = -(-W // ts), -(-H // ts)
     xs, xe, ys, ye = tile_bounds(H, W, ts)
@@ -154,8 +155,12 @@ def bin_lists_np(means2D, radii, rotations, sel, ext, H, W, ts):
     xs1, xe1 = (cx * ts).astype(f32), np.minimum(cx * ts + ts, W).astype(f32)
     ys1, ye1 = (cy * ts).astype(f32), np.minimum(cy * ts + ts, H).astype(f32)
     tiles_of = []
+    T = ext.shape[0]
     with np.errstate(invalid="ignore"):
         for n in range(bb.shape[0]):
+            if not (0 <= sel[n] < T):        # never drawn: in no list
+                tiles_of.append(np.zeros(0, np.int64))
+                continue
             hx = np.nonzero(~((bb[n, 1] < xs1) | (bb[n, 0] > xe1)))[0]
             hy = np.nonzero(~((bb[n, 3] < ys1) | (bb[n, 2] > ye1)))[0]
             tiles_of.append((hy[:, None] * tX + hx[None, :]).ravel())

@@ -165,7 +165,8 @@ static int sample_prim(const dbmp_oracle_scene *s, int n, int px, int py, sample
         const float fty = (v + 1.f) * 0.5f * (float)(Ht - 1);
         if (s->colors_orig && s->c_blend > 0.f)
             color_map(s->colors_orig + (size_t)n * Ht * Wt * 3, Ht, Wt, ftx, fty, o->co);
-        if (ftx < 0.f || ftx > (float)(Wt - 1) || fty < 0.f || fty > (float)(Ht - 1)) return 0;
+        /* NaN coordinates are rejected too (undefined in the reference) */
+        if (!(ftx >= 0.f && ftx <= (float)(Wt - 1) && fty >= 0.f && fty <= (float)(Ht - 1))) return 0;
         x0 = (int)floorf(ftx);
         y0 = (int)floorf(fty);
         wx = (REAL)(ftx - (float)x0);
@@ -191,7 +192,7 @@ static int sample_prim(const dbmp_oracle_scene *s, int n, int px, int py, sample
         const double dty = (v + 1.0) * 0.5 * (Ht - 1);
         if (s->colors_orig && s->c_blend > 0.f)
             color_map(s->colors_orig + (size_t)n * Ht * Wt * 3, Ht, Wt, (REAL)dtx, (REAL)dty, o->co);
-        if (dtx < 0 || dtx > (double)(Wt - 1) || dty < 0 || dty > (double)(Ht - 1)) return 0;
+        if (!(dtx >= 0 && dtx <= (double)(Wt - 1) && dty >= 0 && dty <= (double)(Ht - 1))) return 0;
         x0 = (int)floor(dtx);
         y0 = (int)floor(dty);
         wx = (REAL)(dtx - x0);
@@ -291,8 +292,10 @@ long long oracle_bin_lists(int N, const float *means2D, const float *radii, cons
     float *bb = (float *)malloc(sizeof(float) * 4 * (size_t)(N > 0 ? N : 1));
     oracle_bin_bboxes(N, means2D, radii, rot, sel, ext, T, bb);
     memset(tile_offsets, 0, sizeof(int) * (tiles + 1));
+    /* a primitive whose template index is out of range is never drawn
+     * (tile_forward.cu:211-216,248-276), so it is left out of every list */
     for (int n = 0; n < N; ++n)
-        for (int ty = 0; ty < tY; ++ty) {
+        for (int ty = 0; ty < tY && sel[n] >= 0 && sel[n] < T; ++ty) {
             if (!tile_hit(bb[4 * n + 2], bb[4 * n + 3], ty, ts, H)) continue;
             for (int tx = 0; tx < tX; ++tx)
                 if (tile_hit(bb[4 * n], bb[4 * n + 1], tx, ts, W)) tile_offsets[ty * tX + tx + 1]++;
@@ -303,7 +306,7 @@ long long oracle_bin_lists(int N, const float *means2D, const float *radii, cons
         int *fill = (int *)malloc(sizeof(int) * tiles);
         memcpy(fill, tile_offsets, sizeof(int) * tiles);
         for (int n = 0; n < N; ++n)
-            for (int ty = 0; ty < tY; ++ty) {
+            for (int ty = 0; ty < tY && sel[n] >= 0 && sel[n] < T; ++ty) {
                 if (!tile_hit(bb[4 * n + 2], bb[4 * n + 3], ty, ts, H)) continue;
                 for (int tx = 0; tx < tX; ++tx)
                     if (tile_hit(bb[4 * n], bb[4 * n + 1], tx, ts, W))

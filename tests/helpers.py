@@ -52,8 +52,8 @@ class GpuRun:
         self.alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
         self.T = torch.empty((H, W), dtype=torch.float32, device=dev)
         self.last = torch.empty((H, W), dtype=torch.int32, device=dev)
-        self.h.forward(self.prims, self.bm, self.offs, self.idx, self.eps, self.color, self.alpha,
-                       self.T, self.last)
+        self.h.forward(self.prims, self.bm, _native.Lists(self.offs, self.idx), self.eps, self.color,
+                       self.alpha, self.T, self.last)
         torch.cuda.synchronize()
         return self.color.cpu().numpy(), self.alpha.cpu().numpy()
 
@@ -64,7 +64,7 @@ class GpuRun:
         g = dict(means2D=torch.empty((N, 2), device=dev), radii=torch.empty(N, device=dev),
                  rotations=torch.empty(N, device=dev), opacities=torch.empty(N, device=dev),
                  colors=torch.empty((N, 3), device=dev))
-        self.h.backward(self.prims, self.bm, self.offs, self.idx, self.eps, gCt, gAt, self.color,
+        self.h.backward(self.prims, self.bm, _native.Lists(self.offs, self.idx), self.eps, gCt, gAt, self.color,
                         self.T, self.last, g["means2D"], g["radii"], g["rotations"],
                         g["opacities"], g["colors"])
         torch.cuda.synchronize()

@@ -78,3 +78,85 @@ def test_scale_sampled_tiles(dev):
     err = normwise(g, ref)
     print("sampled-tile normwise gradient error", err)
     assert max(err.values()) <= 1e-4
+
+
+def test_overflow_fallback_is_exact(dev):
+    """Binning into a too-small buffer: the raster kernels walk every
+    primitive instead and produce the same image and gradients."""
+    import torch
+    from diffbmp_amd import _native
+    sc = S.make_scene(0, with_target=False)
+    run = GpuRun(sc, dev, eps_T=1e-6)
+    col, alp = run.forward()
+    gC, gA = rand_grads(sc.H, sc.W, seed=9)
+    g = run.backward(gC, gA)
+    P = run.idx.numel()
+    offs = torch.empty_like(run.offs)
+    idx = torch.empty(P // 3, dtype=torch.int32, device=dev)
+    cnt = torch.empty(1, dtype=torch.int64, device=dev)
+    run.h.bin_async(run.prims, run.ext, tuple(run.d["templates"].shape), P // 3, offs, idx, cnt)
+    assert int(cnt.item()) == P
+    assert int(offs.abs().sum().item()) == 0
+    run.offs, run.idx = offs, idx
+    lists = _native.Lists(offs, idx, cnt, P // 3)
+    H, W = sc.H, sc.W
+    c2 = torch.empty((H, W, 3), device=dev); a2 = torch.empty((H, W), device=dev)
+    T2 = torch.empty((H, W), device=dev); l2 = torch.empty((H, W), dtype=torch.int32, device=dev)
+    run.h.forward(run.prims, run.bm, lists, 1e-6, c2, a2, T2, l2)
+    torch.cuda.synchronize()
+    assert np.abs(c2.cpu().numpy() - col).max() <= 1e-6
+    assert np.abs(a2.cpu().numpy() - alp).max() <= 1e-6
+    N = sc.N
+    gg = [torch.empty(s, device=dev) for s in ((N, 2), (N,), (N,), (N,), (N, 3))]
+    run.h.backward(run.prims, run.bm, lists, 1e-6, torch.from_numpy(gC).to(dev), torch.from_numpy(gA).to(dev),
+                   c2, T2, l2, *gg)
+    torch.cuda.synchronize()
+    for k, t in zip(["means2D", "radii", "rotations", "opacities", "colors"], gg):
+        ref = g[k]
+        assert np.linalg.norm(t.cpu().numpy() - ref) <= 1e-5 * max(np.linalg.norm(ref), 1e-30), k
+
+
+@pytest.mark.parametrize("H,W,ts", [(100, 70, 16), (96, 160, 32), (17, 33, 16)])
+def test_ragged_canvas_and_tile_sizes(dev, H, W, ts):
+    sc = S.make_scene(0, H=H, W=W, N=40, tsz=32, with_target=False)
+    sc.tile_size = ts
+    run = GpuRun(sc, dev, eps_T=0.0, ts=ts)
+    offs, idx = _oracle_lists(sc)
+    assert np.array_equal(run.offs.cpu().numpy(), offs) and np.array_equal(run.idx.cpu().numpy(), idx)
+    col, alp = run.forward()
+    osc = O.scene_from(sc, offs, idx)
+    rc, ra, _ = O.forward(osc)
+    assert max(np.abs(col - rc).max(), np.abs(alp - ra).max()) <= 1e-5
+    gC, gA = rand_grads(H, W, seed=H)
+    g = run.backward(gC, gA)
+    ref = O.backward(osc, gC.astype(np.float64), gA.astype(np.float64))
+    assert max(normwise(g, ref).values()) <= 1e-4
+
+
+def test_empty_and_degenerate_scenes(dev):
+    """No primitives; primitives entirely off canvas; invalid template index;
+    tiny radius below the 1e-6 reciprocal guard."""
+    import copy
+    sc = S.make_scene(0, H=64, W=64, N=12, tsz=16, with_target=False)
+    sc.means2D[:4] = [[-500, 10], [10, -400], [900, 20], [20, 900]]
+    sc.bmp_sel = sc.bmp_sel.copy()
+    sc.bmp_sel[5] = 3             # only one template: invalid, never drawn
+    sc.radii[6] = 1e-7
+    run = GpuRun(sc, dev, eps_T=0.0)
+    col, alp = run.forward()
+    offs, idx = _oracle_lists(sc)
+    assert np.array_equal(run.offs.cpu().numpy(), offs) and np.array_equal(run.idx.cpu().numpy(), idx)
+    osc = O.scene_from(sc, offs, idx)
+    rc, ra, _ = O.forward(osc)
+    assert max(np.abs(col - rc).max(), np.abs(alp - ra).max()) <= 1e-5
+    gC, gA = rand_grads(64, 64, seed=1)
+    g = run.backward(gC, gA)
+    ref = O.backward(osc, gC.astype(np.float64), gA.astype(np.float64))
+    assert max(normwise(g, ref).values()) <= 1e-4
+    assert np.all(g["opacities"][:4] == 0) and np.all(g["colors"][5] == 0)
+    empty = copy.copy(sc)
+    for k in ("means2D", "radii", "rotations", "opacities", "colors", "bmp_sel"):
+        setattr(empty, k, getattr(sc, k)[:0])
+    run0 = GpuRun(empty, dev)
+    col0, alp0 = run0.forward()
+    assert run0.idx.numel() == 0 and col0.max() == 0 and alp0.max() == 0
