@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
     ap.add_argument("--eps-t", type=float, default=None)
+    ap.add_argument("--mode", default="fused", choices=["fused", "autograd"],
+                    help="fused: the device-resident FitJob step (binning, forward, loss, backward, "
+                         "Adam as HIP kernels); autograd: the reference-compatible TileRasterizer "
+                         "behind torch autograd and torch.optim.Adam")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=48)
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -90,6 +94,99 @@ def cpu_baseline(scene, ntiles_sample):
                        f"{step_s * 1e3:.0f} ms per full step")
 
 
+def make_problem(sc, dev, sdt):
+    import torch
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    return dict(params=[t(a) for a in (sc.means2D, sc.radii, sc.rotations, sc.opacities, sc.colors)],
+                templates=t(sc.templates).to(sdt), sel=t(sc.bmp_sel), target=t(sc.target))
+
+
+def run_fused(args, sc, dev, sdt):
+    """Device-resident FitJob: every stage of the step is a HIP kernel."""
+    import torch
+    from diffbmp_amd.fit import FitJob
+    pr = make_problem(sc, dev, sdt)
+    job = FitJob(*pr["params"], pr["templates"], pr["sel"], pr["target"], lr=1e-2, store_dtype=sdt,
+                 max_steps=args.warmup + 2 * args.steps + 16)
+    if args.eps_t is not None:
+        job.eps_T = args.eps_t
+    return job, job.step, job.handle
+
+
+def run_autograd(args, sc, dev, sdt):
+    """The reference-compatible TileRasterizer behind torch autograd."""
+    import torch
+    import torch.nn.functional as F
+    from diffbmp_amd.tile_rasterizer import TileRasterizer
+    pr = make_problem(sc, dev, sdt)
+    leaves = [p.clone().requires_grad_() for p in pr["params"]]
+    H, W, N = sc.H, sc.W, sc.N
+    rast = TileRasterizer(H, W, 16, 0.0, 1.0, 151, N, use_fp16=(sdt != torch.float32), storage_dtype=sdt)
+    if args.eps_t is not None:
+        rast.eps_T = args.eps_t
+    opt = torch.optim.Adam(leaves, lr=1e-2)
+    bg = torch.ones((H, W, 3), device=dev)
+    empty = torch.empty(0, device=dev)
+    lr_conf = torch.tensor([1e-2, 1, 1, 1, 1, 1, 1], device=dev)
+    rmax = float(min(H, W) // 4)
+    templates, sel, target = pr["templates"], pr["sel"], pr["target"]
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        color, alpha = rast(*leaves, empty, templates, sel, 0.0, lr_conf, None)
+        loss = F.mse_loss(color.float() + (1.0 - alpha.float()).unsqueeze(-1) * bg, target)
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            leaves[0][:, 0].clamp_(0.0, float(W - 1))
+            leaves[0][:, 1].clamp_(0.0, float(H - 1))
+            leaves[1].clamp_(2.0, rmax)
+
+    return rast, step, rast.handle
+
+
+def time_mode(args, runner, sc, dev, sdt):
+    import torch
+    obj, step, handle = runner(args, sc, dev, sdt)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    handle.timing_reset()
+    handle.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    handle.timing_enable(False)
+    f_ms, f_n, b_ms, b_n = handle.timing_stats()
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    lat = []
+    for _ in range(min(args.steps, 100)):
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        e1.synchronize()
+        lat.append(e0.elapsed_time(e1))
+    return dict(obj=obj, handle=handle, total=t1 - t0, lat=np.array(lat),
+                fwd_s=f_ms / max(f_n, 1) * 1e-3, bwd_s=b_ms / max(b_n, 1) * 1e-3)
+
+
+def list_stats(handle, sc, dev, sdt, eps_T):
+    """Pair count and list-walk depth at the initial parameters."""
+    import torch
+    from diffbmp_amd.tile_rasterizer import _Run
+    pr = make_problem(sc, dev, sdt)
+    run = _Run(handle, sdt, *pr["params"], None, pr["templates"], pr["sel"], 0.0, None, eps_T)
+    _, _, _, last = run.forward()
+    torch.cuda.synchronize()
+    offs = run.lists.offsets.long()
+    lens = (offs[1:] - offs[:-1]).float()
+    return int(offs[-1].item()), float(lens.mean().item()), float(last.float().mean().item())
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -97,9 +194,7 @@ def main():
               "path has no multi-GPU mode in this build.", file=sys.stderr)
         sys.exit(2)
     import torch
-    import torch.nn.functional as F
     from diffbmp_amd import scenes as S
-    from diffbmp_amd.tile_rasterizer import TileRasterizer, DEFAULT_EPS_T
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -107,117 +202,47 @@ def main():
         nb = args.copy_probe << 20
         src = torch.ones(nb // 4, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     sc = S.make_scene(1)
     H, W, N = sc.H, sc.W, sc.N
     T, Ht, Wt = sc.templates.shape
     sdt = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[args.precision]
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    means2D = t(sc.means2D).requires_grad_()
-    radii = t(sc.radii).requires_grad_()
-    rot = t(sc.rotations).requires_grad_()
-    op = t(sc.opacities).requires_grad_()
-    col = t(sc.colors).requires_grad_()
-    templates = t(sc.templates).to(sdt)
-    sel = t(sc.bmp_sel)
-    target = t(sc.target)
-    bg = torch.ones((H, W, 3), device=dev)
-    empty = torch.empty(0, device=dev)
-    lr_conf = torch.tensor([1e-2, 1, 1, 1, 1, 1, 1], device=dev)
-    rast = TileRasterizer(H, W, 16, 0.0, 1.0, 151, N, use_fp16=(args.precision != "fp32"),
-                          storage_dtype=sdt)
-    if args.eps_t is not None:
-        rast.eps_T = args.eps_t
-    opt = torch.optim.Adam([means2D, radii, rot, op, col], lr=1e-2)
-    rmax = float(min(H, W) // 4)
+    runners = {"fused": run_fused, "autograd": run_autograd}
+    res = time_mode(args, runners[args.mode], sc, dev, sdt)
+    other = "autograd" if args.mode == "fused" else "fused"
+    res_other = time_mode(args, runners[other], sc, dev, sdt)
+    eps_T = args.eps_t if args.eps_t is not None else 1e-6
+    P, list_mean, walked = list_stats(res["handle"], sc, dev, sdt, eps_T)
 
-    stream = torch.cuda.current_stream(dev)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-
-    def step(record):
-        opt.zero_grad(set_to_none=True)
-        if record:
-            ev[0].record(stream)
-        color, alpha = rast(means2D, radii, rot, op, col, empty, templates, sel, 0.0, lr_conf, None)
-        result = color.float() + (1.0 - alpha.float()).unsqueeze(-1) * bg
-        loss = F.mse_loss(result, target)
-        loss.backward()
-        if record:
-            ev[1].record(stream)
-        opt.step()
-        with torch.no_grad():
-            means2D[:, 0].clamp_(0.0, float(W - 1))
-            means2D[:, 1].clamp_(0.0, float(H - 1))
-            radii.clamp_(2.0, rmax)
-        return loss
-
-    for _ in range(args.warmup):
-        step(False)
-    torch.cuda.synchronize()
-
-    # per-kernel (raster launch) timing inside the library, hipEvent on the launch stream
-    rast.handle.timing_reset()
-    rast.handle.timing_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(False)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    rast.handle.timing_enable(False)
-    f_ms, f_n, b_ms, b_n = rast.handle.timing_stats()
-
-    # a second, shorter pass for per-step fwd+bwd latency percentiles
-    lat = []
-    for _ in range(min(args.steps, 100)):
-        step(True)
-        ev[1].synchronize()
-        lat.append(ev[0].elapsed_time(ev[1]))
-    lat = np.array(lat)
-
-    # pair count and list-walk depth of the final state
-    with torch.no_grad():
-        from diffbmp_amd.tile_rasterizer import _Run
-        run = _Run(rast.handle, sdt, means2D.detach(), radii.detach(), rot.detach(), op.detach(),
-                   col.detach(), None, templates, sel, 0.0, None, rast.eps_T)
-        _, _, _, last = run.forward()
-        torch.cuda.synchronize()
-        offs = run.lists.offsets.long()
-        P = int(offs[-1].item())
-        lens = (offs[1:] - offs[:-1]).float()
-        walked = float(last.float().mean().item())
-        list_mean = float(lens.mean().item())
     b_p = torch.tensor([], dtype=sdt).element_size()
-    b_t = b_p
-    b_bin, b_fwd, b_bwd = algorithmic_bytes(H, W, N, T, Ht, Wt, P, rast.handle.tiles, b_p, b_t)
-    total = t1 - t0
-    ms_step = total / args.steps * 1e3
-    bwd_avg = b_ms / max(b_n, 1) * 1e-3
-    fwd_avg = f_ms / max(f_n, 1) * 1e-3
-    achieved = b_bwd / bwd_avg
+    b_bin, b_fwd, b_bwd = algorithmic_bytes(H, W, N, T, Ht, Wt, P, res["handle"].tiles, b_p, b_p)
+    achieved = b_bwd / res["bwd_s"]
+    lat = res["lat"]
     out = {
         "metric": "fwd+bwd iters/s at 1024^2, 8192 prims",
-        "value": round(args.steps / total, 3),
+        "value": round(args.steps / res["total"], 3),
         "unit": "iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(res["total"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp32": "f32", "fp16": "f16-store/f32-acc", "bf16": "bf16-store/f32-acc"}[args.precision],
         "data": "synthetic",
         "config": {"workload": "scale: 1024x1024 RGB, 8192 primitives, 4x128^2 bitmaps, 16x16 tiles, "
-                               "Adam lr 1e-2 on MSE to a smooth random field",
+                               "Adam lr 1e-2 on MSE to a smooth random field; one step = binning + "
+                               "forward + composite/MSE + backward + Adam + clamp",
                    "canvas": [H, W], "primitives": N, "bitmaps": [T, Ht, Wt], "tile": 16,
-                   "eps_T": rast.eps_T, "precision": args.precision},
-        "fwd_bwd_ms_p50": round(float(np.percentile(lat, 50)), 4),
-        "fwd_bwd_ms_p99": round(float(np.percentile(lat, 99)), 4),
-        "raster_fwd_ms": round(fwd_avg * 1e3, 4),
-        "raster_bwd_ms": round(bwd_avg * 1e3, 4),
+                   "eps_T": eps_T, "precision": args.precision, "mode": args.mode},
+        "step_ms_p50": round(float(np.percentile(lat, 50)), 4),
+        "step_ms_p99": round(float(np.percentile(lat, 99)), 4),
+        "raster_fwd_ms": round(res["fwd_s"] * 1e3, 4),
+        "raster_bwd_ms": round(res["bwd_s"] * 1e3, 4),
+        f"{other}_iters_per_s": round(args.steps / res_other["total"], 3),
         "pairs": P,
         "list_len_mean": round(list_mean, 2),
         "entries_walked_per_pixel": round(walked, 2),

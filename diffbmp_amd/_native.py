@@ -27,7 +27,22 @@ class Prims(ctypes.Structure):
     _fields_ = [("means2D", ctypes.c_void_p), ("radii", ctypes.c_void_p),
                 ("rotations", ctypes.c_void_p), ("opacities", ctypes.c_void_p),
                 ("colors", ctypes.c_void_p), ("bmp_sel", ctypes.c_void_p),
-                ("num_primitives", ctypes.c_int32), ("dtype", ctypes.c_int32)]
+                ("num_primitives", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("reuse_records", ctypes.c_int32)]
+
+
+class AdamGroup(ctypes.Structure):
+    _fields_ = [("param", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
+                ("exp_avg_sq", ctypes.c_void_p), ("grad", ctypes.c_void_p),
+                ("store", ctypes.c_void_p), ("numel", ctypes.c_int64), ("lr", ctypes.c_float),
+                ("stride", ctypes.c_int32), ("clamp", ctypes.c_int32),
+                ("lo", ctypes.c_float * 2), ("hi", ctypes.c_float * 2)]
+
+
+class Adam(ctypes.Structure):
+    _fields_ = [("groups", AdamGroup * 5), ("num_groups", ctypes.c_int32),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+                ("step", ctypes.c_int64), ("store_dtype", ctypes.c_int32)]
 
 
 class Bitmaps(ctypes.Structure):
@@ -82,6 +97,11 @@ EXPORTS = {
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_loss_mse": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]),
+    "dbmp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Adam), ctypes.c_void_p]),
     "dbmp_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dbmp_timing_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_int64),
@@ -172,7 +192,7 @@ class Handle:
 
     # ------------------------------------------------------------------
     @staticmethod
-    def prims(means2D, radii, rotations, opacities, colors, bmp_sel) -> Prims:
+    def prims(means2D, radii, rotations, opacities, colors, bmp_sel, reuse: bool = False) -> Prims:
         dt = _DT[means2D.dtype]
         for t, nm in ((radii, "radii"), (rotations, "rotations"), (opacities, "opacities"),
                       (colors, "colors")):
@@ -186,7 +206,7 @@ class Handle:
         if bmp_sel.dtype != torch.int32:
             raise TypeError("bmp_sel must be int32")
         return Prims(means2D.data_ptr(), radii.data_ptr(), rotations.data_ptr(),
-                     opacities.data_ptr(), colors.data_ptr(), bmp_sel.data_ptr(), N, dt)
+                     opacities.data_ptr(), colors.data_ptr(), bmp_sel.data_ptr(), N, dt, int(reuse))
 
     @staticmethod
     def bitmaps(templates, colors_orig=None, colors_orig_mode=0, c_blend=0.0,
@@ -304,6 +324,15 @@ class Handle:
                                     float(eps_T), _ptr(grad_color), _ptr(grad_alpha), _ptr(out_color),
                                     _ptr(out_T), _ptr(out_last), _ptr(g_means), _ptr(g_r), _ptr(g_th),
                                     _ptr(g_v), _ptr(g_c), _stream(grad_color.device)))
+
+    def loss_mse(self, color, alpha, target, grad_color, grad_alpha, loss_accum=None, bg=None,
+                 bg_value: float = 1.0) -> None:
+        check(self._L.dbmp_loss_mse(self._h, _ptr(color), _ptr(alpha), _ptr(bg), float(bg_value),
+                                    _ptr(target), _ptr(grad_color), _ptr(grad_alpha), _ptr(loss_accum),
+                                    _stream(color.device)))
+
+    def adam_step(self, cfg: "Adam", device) -> None:
+        check(self._L.dbmp_adam_step(self._h, ctypes.byref(cfg), _stream(device)))
 
     # timing -------------------------------------------------------------
     def timing_enable(self, on: bool = True) -> None:

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,7 @@
 #include "../../include/diffbmp_amd.h"
 #include "bin_kernels.h"
 #include "raster_kernels.h"
+#include "step_kernels.h"
 
 using namespace dbmp;
 
@@ -104,6 +106,7 @@ struct dbmp_rasterizer {
   int device = 0;
   // primitive-sized scratch
   int n_cap = 0;
+  int recs_n = -1;   // N of the records currently in `recs` (-1: none)
   PrimRec* recs = nullptr;
   int* counts = nullptr;
   long long* offs = nullptr;
@@ -201,6 +204,8 @@ int check_bitmaps(const dbmp_bitmaps* b) {
 int launch_prep(dbmp_rasterizer* h, const dbmp_prims* p, const float* ext, int T, int Ht, int Wt,
                 hipStream_t st) {
   const int N = p->num_primitives;
+  if (ext == nullptr && p->reuse_records && h->recs_n == N) return 0;
+  h->recs_n = N;
   if (N == 0) return 0;
   const dim3 grid(ceil_div(N, 256)), block(256);
 #define PREP_ARGS(TY)                                                                          \
@@ -517,6 +522,59 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
     LAUNCH_CHECK("k_unpack_grads");
   }
   timing_end(h, h->ev_bwd, st);
+  return 0;
+}
+
+int dbmp_loss_mse(dbmp_rasterizer* h, const float* color, const float* alpha, const float* bg,
+                  float bg_value, const float* target, float* grad_color, float* grad_alpha,
+                  float* loss_accum, void* stream) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  if (!color || !alpha || !target || !grad_color || !grad_alpha)
+    return fail(DBMP_E_ARG, "NULL image pointer");
+  const long long npix = (long long)h->H * h->W;
+  const float inv = 1.f / (float)(3 * npix);
+  const int grid = (int)std::min<long long>(ceil_div(npix, 256), 2048);
+  hipLaunchKernelGGL(k_loss_mse, dim3(grid), dim3(256), 0, (hipStream_t)stream, color, alpha, bg,
+                     bg_value, target, npix, inv, grad_color, grad_alpha, loss_accum);
+  LAUNCH_CHECK("k_loss_mse");
+  return 0;
+}
+
+int dbmp_adam_step(dbmp_rasterizer* h, const dbmp_adam* cfg, void* stream) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  if (!cfg || cfg->num_groups < 1 || cfg->num_groups > 5) return fail(DBMP_E_ARG, "bad Adam config");
+  if (cfg->step < 1) return fail(DBMP_E_ARG, "Adam step must be >= 1");
+  if (cfg->store_dtype < DBMP_F32 || cfg->store_dtype > DBMP_BF16) return fail(DBMP_E_ARG, "bad store dtype");
+  AdamArgs A;
+  long long nmax = 0;
+  const double bc1 = 1.0 - std::pow((double)cfg->beta1, (double)cfg->step);
+  const double bc2 = 1.0 - std::pow((double)cfg->beta2, (double)cfg->step);
+  for (int i = 0; i < 5; ++i) {
+    AdamGroup& g = A.g[i];
+    if (i < cfg->num_groups) {
+      const dbmp_adam_group& c = cfg->groups[i];
+      if (c.numel > 0 && (!c.param || !c.exp_avg || !c.exp_avg_sq || !c.grad))
+        return fail(DBMP_E_ARG, "Adam group %d has a NULL pointer", i);
+      g.p = c.param; g.m = c.exp_avg; g.v = c.exp_avg_sq; g.g = c.grad; g.store = c.store;
+      g.n = c.numel; g.stride = c.stride; g.clamp = c.clamp;
+      g.lo[0] = c.lo[0]; g.lo[1] = c.lo[1]; g.hi[0] = c.hi[0]; g.hi[1] = c.hi[1];
+      A.step_scale[i] = (float)((double)c.lr / bc1);
+      nmax = std::max(nmax, (long long)c.numel);
+    } else {
+      g = AdamGroup{};
+      g.n = 0;
+      A.step_scale[i] = 0.f;
+    }
+  }
+  A.beta1 = cfg->beta1;
+  A.beta2 = cfg->beta2;
+  A.eps = cfg->eps;
+  A.inv_bc2_sqrt = (float)(1.0 / std::sqrt(bc2));
+  A.store_dtype = cfg->store_dtype;
+  if (nmax == 0) return 0;
+  const int gx = (int)std::min<long long>(ceil_div(nmax, 256), 1024);
+  hipLaunchKernelGGL(k_adam, dim3(gx, cfg->num_groups), dim3(256), 0, (hipStream_t)stream, A);
+  LAUNCH_CHECK("k_adam");
   return 0;
 }
 

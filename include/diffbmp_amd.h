@@ -41,7 +41,10 @@ extern "C" {
 
 typedef struct dbmp_rasterizer dbmp_rasterizer;
 
-/* Per-primitive parameters, storage dtype DBMP_F32 / F16 / BF16. */
+/* Per-primitive parameters, storage dtype DBMP_F32 / F16 / BF16.
+ * reuse_records: nonzero asserts that these exact parameter values were the
+ * input of the previous dbmp_bin / dbmp_forward / dbmp_backward call on this
+ * handle, so the per-primitive constants computed there are reused. */
 typedef struct {
   const void* means2D;     /* (N,2) */
   const void* radii;       /* (N)   */
@@ -51,6 +54,7 @@ typedef struct {
   const int32_t* bmp_sel;  /* (N) template index */
   int32_t num_primitives;
   int32_t dtype;
+  int32_t reuse_records;
 } dbmp_prims;
 
 /* Alpha bitmaps and optional colour maps (same dtype as the bitmaps). */
@@ -132,6 +136,44 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
                   const float* grad_alpha, const float* out_color, const float* out_T_final,
                   const int32_t* out_last, float* grad_means2D, float* grad_radii,
                   float* grad_rotations, float* grad_opacities, float* grad_colors, void* stream);
+
+/* Background composite + mean squared error + gradient, one kernel:
+ *   result = color + (1 - alpha) * bg,  loss = mean((result - target)^2),
+ *   grad_color = dloss/dcolor (H,W,3), grad_alpha = dloss/dalpha (H,W);
+ * loss is added to *loss_accum (device float).  bg is (H,W,3), or NULL for the
+ * constant bg_value (1 = white).  Replaces the torch composite and loss of
+ * simple_tile_renderer.py:703-707 and loss_functions.py:20-40 together with
+ * their autograd backward. */
+int dbmp_loss_mse(dbmp_rasterizer* h, const float* color, const float* alpha, const float* bg,
+                  float bg_value, const float* target, float* grad_color, float* grad_alpha,
+                  float* loss_accum, void* stream);
+
+/* One parameter group of dbmp_adam_step (fp32 master weights). */
+typedef struct {
+  float* param;
+  float* exp_avg;
+  float* exp_avg_sq;
+  const float* grad;
+  void* store;          /* optional copy written in store_dtype (or NULL) */
+  int64_t numel;
+  float lr;
+  int32_t stride;       /* 2 for means2D: element i uses lo[i%2], hi[i%2] */
+  int32_t clamp;        /* clamp to [lo, hi] after the update */
+  float lo[2], hi[2];
+} dbmp_adam_group;
+
+typedef struct {
+  dbmp_adam_group groups[5];
+  int32_t num_groups;
+  float beta1, beta2, eps;
+  int64_t step;         /* 1-based count including this update */
+  int32_t store_dtype;
+} dbmp_adam;
+
+/* torch.optim.Adam update (no weight decay, no amsgrad) of up to five groups
+ * plus the reference's clamp x in [0,W-1], y in [0,H-1], r in [2, min(H,W)/4]
+ * (simple_tile_renderer.py:122-128, 961-968, 1125-1131), one kernel. */
+int dbmp_adam_step(dbmp_rasterizer* h, const dbmp_adam* cfg, void* stream);
 
 /* Timing: hipEvent brackets around each forward/backward launch sequence;
  * replaces print_/reset_cuda_timing_stats (ext.cpp:110-121,
