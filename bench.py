@@ -184,7 +184,8 @@ def list_stats(handle, sc, dev, sdt, eps_T):
     torch.cuda.synchronize()
     offs = run.lists.offsets.long()
     lens = (offs[1:] - offs[:-1]).float()
-    return int(offs[-1].item()), float(lens.mean().item()), float(last.float().mean().item())
+    walked = (last & 0x3FFFFFFF).float().mean().item()
+    return int(offs[-1].item()), float(lens.mean().item()), float(walked)
 
 
 def main():

@@ -14,7 +14,10 @@
 // tile_backward.cu:101-452) with two documented changes:
 //   * early termination: a pixel stops once T < eps_T (eps_T = 0 disables it);
 //     the dropped colour is below eps_T.  The forward records per pixel the
-//     number of list entries consumed ("last") and T_final;
+//     number of list entries consumed ("last") and the final transmittance,
+//     kept as the product of the nonzero (1 - a) factors plus a count of
+//     exactly opaque layers so that d alpha / d a of an opaque layer (the
+//     product of the layers behind it) stays exact;
 //   * the backward never stores per-layer state: it replays the list in
 //     forward order, recomputing T_k exactly as the forward did, takes the
 //     suffix colour as C_final - C_prefix and the suffix transmittance as
@@ -207,7 +210,10 @@ __global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __
   const BlockGeom G = block_geom(A);
   const float fpx = (float)G.px, fpy = (float)G.py;
   bool alive = (G.px < A.W) && (G.py < A.H);
-  float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+  // T: transmittance; Tnz: product of the nonzero (1 - a) factors; nz: count
+  // of zero factors (exactly opaque layers), capped at 2
+  float T = 1.f, Tnz = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+  int nz = 0;
   int last = -1;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
@@ -236,8 +242,13 @@ __global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __
         C0 = fmaf(wgt, col[0], C0);
         C1 = fmaf(wgt, col[1], C1);
         C2 = fmaf(wgt, col[2], C2);
-        T = T * fmaxf(1.f - a, 0.f);
-        const bool dead = S.pushed && T < A.eps;
+        const float f = fmaxf(1.f - a, 0.f);
+        T = T * f;
+        Tnz = f > 0.f ? Tnz * f : Tnz;
+        nz = f > 0.f ? nz : min(nz + 1, 2);
+        // a pixel stops once later layers can change neither its colour nor
+        // its alpha (or alpha's derivative w.r.t. an opaque layer) by eps_T
+        const bool dead = S.pushed && (Tnz < A.eps || nz >= 2);
         last = dead ? base - start + j + 1 : last;
         alive = alive && !dead;
         if (!__any(alive)) break;
@@ -252,8 +263,8 @@ __global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __
     out_color[3 * p + 1] = C1;
     out_color[3 * p + 2] = C2;
     out_alpha[p] = 1.f - T;
-    out_T[p] = T;
-    out_last[p] = last < 0 ? end - start : last;
+    out_T[p] = Tnz;
+    out_last[p] = (last < 0 ? end - start : last) | (nz << 30);
   }
 }
 
@@ -274,13 +285,16 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
   const float fpx = (float)G.px, fpy = (float)G.py;
   const bool okpix = (G.px < A.W) && (G.py < A.H);
   const long pix = okpix ? (long)G.py * A.W + G.px : 0;
-  const int lst = okpix ? lastin[pix] : 0;
+  const int lstw = okpix ? lastin[pix] : 0;
+  const int lst = lstw & 0x3fffffff;   // list entries consumed by the forward
+  const int nzf = (unsigned)lstw >> 30; // exactly opaque layers seen (capped at 2)
   const float gc0 = okpix ? gCin[3 * pix] : 0.f, gc1 = okpix ? gCin[3 * pix + 1] : 0.f,
               gc2 = okpix ? gCin[3 * pix + 2] : 0.f;
   const float ga = (okpix && gAin) ? gAin[pix] : 0.f;
   const float cf0 = okpix ? Cfin[3 * pix] : 0.f, cf1 = okpix ? Cfin[3 * pix + 1] : 0.f,
               cf2 = okpix ? Cfin[3 * pix + 2] : 0.f;
-  const float tf = okpix ? Tfin[pix] : 0.f;
+  const float tnzf = okpix ? Tfin[pix] : 0.f;   // product of the nonzero (1 - a) factors
+  const float tf = nzf ? 0.f : tnzf;            // transmittance after the last layer
   float T = 1.f, ca0 = 0.f, ca1 = 0.f, ca2 = 0.f;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
@@ -323,9 +337,15 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
       const float S0 = cf0 - ca0, S1 = cf1 - ca1, S2 = cf2 - ca2;
       const float om = 1.f - a;
       const float inv1m = __builtin_amdgcn_rcpf(fmaxf(om, 1e-6f));
-      const float Tn = Tk * fmaxf(om, 0.f);
-      // suffix transmittance B_k = T_final / T_{k+1}; 1 if nothing follows
-      const float B = Tn > 0.f ? fminf(tf * __builtin_amdgcn_rcpf(Tn), 1.f) : 1.f;
+      const float fk = fmaxf(om, 0.f);
+      const float Tn = Tk * fk;
+      // suffix transmittance B_k = prod_{m>k} (1 - a_m) = T_final / T_{k+1};
+      // for an exactly opaque layer (fk = 0) it is the nonzero-factor product
+      // after it, tnzf / T_k, if it is the only opaque layer, else 0
+      float B;
+      if (Tn > 0.f) B = fminf(tf * __builtin_amdgcn_rcpf(Tn), 1.f);
+      else if (fk == 0.f) B = nzf == 1 ? fminf(tnzf * __builtin_amdgcn_rcpf(Tk), 1.f) : 0.f;
+      else B = 1.f;
       float dLda = ga * (Tk * B);
       dLda = fmaf(gc0, fmaf(Tk, col[0], -S0 * inv1m), dLda);
       dLda = fmaf(gc1, fmaf(Tk, col[1], -S1 * inv1m), dLda);
