@@ -46,6 +46,9 @@ def parse():
                          "behind torch autograd and torch.optim.Adam")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=48)
+    ap.add_argument("--psnr-steps", type=int, default=0,
+                    help="also fit this many Adam steps at fp32 and at --precision from the same "
+                         "start and report both final PSNRs")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "reference kernel for scaling HBM byte counters)")
@@ -174,6 +177,27 @@ def time_mode(args, runner, sc, dev, sdt):
                 fwd_s=f_ms / max(f_n, 1) * 1e-3, bwd_s=b_ms / max(b_n, 1) * 1e-3)
 
 
+def psnr_compare(args, sc, dev, sdt):
+    """Fit `psnr_steps` Adam steps from the same start at fp32 and at the chosen
+    storage precision; PSNR (dB) of the final render against the target."""
+    import torch
+    from diffbmp_amd.fit import FitJob
+    res = {}
+    for name, dt in (("fp32", torch.float32), (args.precision, sdt)):
+        if name in res:
+            continue
+        pr = make_problem(sc, dev, dt)
+        job = FitJob(*pr["params"], pr["templates"], pr["sel"], pr["target"], lr=1e-2, store_dtype=dt,
+                     max_steps=args.psnr_steps + 1)
+        for _ in range(args.psnr_steps):
+            job.step()
+        img = job.render()
+        mse = float(((img - pr["target"]) ** 2).mean().item())
+        res[name] = round(FitJob.psnr(mse), 4)
+    res["steps"] = args.psnr_steps
+    return res
+
+
 def list_stats(handle, sc, dev, sdt, eps_T):
     """Pair count and list-walk depth at the initial parameters."""
     import torch
@@ -252,6 +276,8 @@ def main():
                      "kernel": "k_raster_bwd", "algorithmic_bytes": int(b_bwd),
                      "step_algorithmic_bytes": int(b_bin + b_fwd + b_bwd)},
     }
+    if args.psnr_steps:
+        out["psnr"] = psnr_compare(args, sc, dev, sdt)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_tiles)
     print(json.dumps(out))

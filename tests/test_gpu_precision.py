@@ -1,0 +1,73 @@
+"""Precision form (fp16 / bf16 storage, fp32 arithmetic) against the FP64 oracle.
+
+For the 1024^2 / 8192-primitive workload, on a random sample of tiles:
+  * arithmetic error: HIP vs the oracle run on the same rounded inputs --
+    must meet the fp32 bar (image <= 1e-5, gradients <= 1e-4 normwise);
+  * total error: HIP vs the oracle on the original fp32 inputs -- adds the
+    storage quantisation (fp16 positions move in 0.5 px steps at 512-1024 px,
+    bf16 in 4 px steps; the reference's fp16 path casts means2D to half too,
+    cuda_tile_rasterizer/__init__.py:348).  Reported, bounded loosely.
+The error table is printed and written to gpurun_out/precision_errors.json
+when that folder exists.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from diffbmp_amd import scenes as S
+from helpers import GpuRun, rand_grads, rounded
+
+pytestmark = pytest.mark.gpu
+
+GRADS = ("means2D", "radii", "rotations", "opacities", "colors")
+
+
+def _errs(img, ref_img, g, ref_g, pm):
+    out = {"image_max_abs": float(np.abs(img - ref_img)[pm].max()),
+           "image_mean_abs": float(np.abs(img - ref_img)[pm].mean())}
+    for k in GRADS:
+        d = g[k] - ref_g[k]
+        out[f"grad_{k}_normwise"] = float(np.linalg.norm(d) / max(np.linalg.norm(ref_g[k]), 1e-30))
+        out[f"grad_{k}_max_abs"] = float(np.abs(d).max())
+    return out
+
+
+@pytest.mark.parametrize("store", ["fp16", "bf16", "fp32"])
+def test_precision_error_table(dev, store):
+    dt = {"fp16": torch.float16, "bf16": torch.bfloat16, "fp32": torch.float32}[store]
+    sc = S.make_scene(2, with_target=False)
+    run = GpuRun(sc, dev, dtype=dt, eps_T=1e-6)
+    col, alp = run.forward()
+    rng = np.random.default_rng(17)
+    tiles = np.sort(rng.choice(64 * 64, 16, replace=False)).astype(np.int32)
+    mask = np.zeros((64, 64), bool)
+    mask.flat[tiles] = True
+    pm = np.kron(mask, np.ones((16, 16), bool))
+    gC, gA = rand_grads(sc.H, sc.W, seed=23, scale=1e-3)
+    gC[~pm] = 0
+    gA[~pm] = 0
+    g = run.backward(gC, gA)
+    img = np.concatenate([col, alp[..., None]], -1)
+    table = {}
+    offs, idx = run.offs.cpu().numpy(), run.idx.cpu().numpy()
+    for name, scn in (("arithmetic", rounded(sc, dt)), ("total", sc)):
+        osc = O.scene_from(scn, offs, idx, tile_subset=tiles)
+        rc, ra, _ = O.forward(osc)
+        ref_g = O.backward(osc, gC.astype(np.float64), gA.astype(np.float64))
+        ref_img = np.concatenate([rc, ra[..., None]], -1)
+        table[name] = _errs(img, ref_img, g, ref_g, pm)
+    print(store, json.dumps(table, indent=1))
+    if os.path.isdir("gpurun_out"):
+        path = "gpurun_out/precision_errors.json"
+        allt = json.load(open(path)) if os.path.exists(path) else {}
+        allt[store] = table
+        json.dump(allt, open(path, "w"), indent=1)
+    a = table["arithmetic"]
+    assert a["image_max_abs"] <= 1e-5
+    assert max(a[f"grad_{k}_normwise"] for k in GRADS) <= 1e-4
+    if store == "fp16":
+        assert table["total"]["image_mean_abs"] <= 2e-3
