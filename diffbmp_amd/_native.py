@@ -50,7 +50,7 @@ class Bitmaps(ctypes.Structure):
                 ("num_templates", ctypes.c_int32), ("height", ctypes.c_int32),
                 ("width", ctypes.c_int32), ("dtype", ctypes.c_int32),
                 ("colors_orig_mode", ctypes.c_int32), ("c_blend", ctypes.c_float),
-                ("version", ctypes.c_int64)]
+                ("version", ctypes.c_int64), ("blend_grad_exact", ctypes.c_int32)]
 
 
 class TileLists(ctypes.Structure):
@@ -210,7 +210,7 @@ class Handle:
 
     @staticmethod
     def bitmaps(templates, colors_orig=None, colors_orig_mode=0, c_blend=0.0,
-                version: int = -1) -> Bitmaps:
+                version: int = -1, blend_grad_exact: bool = False) -> Bitmaps:
         if templates.dim() != 3:
             raise ValueError("templates must be (T,Ht,Wt)")
         T, Ht, Wt = templates.shape
@@ -222,7 +222,8 @@ class Handle:
                 raise ValueError("colors_orig must be (M,Ht,Wt,3)")
             co = colors_orig.data_ptr()
         return Bitmaps(templates.data_ptr(), co, T, Ht, Wt, _DT[templates.dtype],
-                       int(colors_orig_mode if co else 0), float(c_blend), int(version))
+                       int(colors_orig_mode if co else 0), float(c_blend), int(version),
+                       int(bool(blend_grad_exact)))
 
     def template_extents(self, templates: torch.Tensor) -> torch.Tensor:
         require_gpu(templates, "templates")

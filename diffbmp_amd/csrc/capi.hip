@@ -268,6 +268,7 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.cmap = blend ? b->colors_orig : nullptr;
   A.cmap_per_prim = b->colors_orig_mode == 1;
   A.c_blend = b->c_blend;
+  A.exact_blend = b->blend_grad_exact != 0;
   A.N = p->num_primitives;
   A.Ht = b->height;
   A.Wt = b->width;

@@ -150,6 +150,31 @@ __device__ __forceinline__ void color_map_sample(const CT* cm, int Ht, int Wt, f
   }
 }
 
+// The same colour-map sample and its derivative w.r.t. (cx, cy); zero in a
+// direction where the coordinate sits on the last texel (clamp-to-edge).
+template <typename CT>
+__device__ __forceinline__ void color_map_sample_grad(const CT* cm, int Ht, int Wt, float cx,
+                                                      float cy, float out[3], float dcx[3],
+                                                      float dcy[3]) {
+  cx = fmaxf(0.f, fminf((float)(Wt - 1), cx));
+  cy = fmaxf(0.f, fminf((float)(Ht - 1), cy));
+  const int x0 = (int)cx, y0 = (int)cy;
+  const int x1 = min(x0 + 1, Wt - 1), y1 = min(y0 + 1, Ht - 1);
+  const float fx = cx - (float)x0, fy = cy - (float)y0;
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    const float c00 = ldf(cm, 3L * (y0 * Wt + x0) + ch);
+    const float c01 = ldf(cm, 3L * (y1 * Wt + x0) + ch);
+    const float c10 = ldf(cm, 3L * (y0 * Wt + x1) + ch);
+    const float c11 = ldf(cm, 3L * (y1 * Wt + x1) + ch);
+    const float c0 = fmaf(fx, c10 - c00, c00);
+    const float c1 = fmaf(fx, c11 - c01, c01);
+    out[ch] = fmaf(fy, c1 - c0, c0);
+    dcx[ch] = fmaf(fy, (c11 - c01) - (c10 - c00), c10 - c00);
+    dcy[ch] = c1 - c0;
+  }
+}
+
 // Wave-level helpers (wave64) ---------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

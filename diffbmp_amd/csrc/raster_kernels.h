@@ -39,6 +39,8 @@ struct RasterArgs {
   const void* cmap;      // colour maps (M,Ht,Wt,3) or null
   int cmap_per_prim;     // 1: M = N (indexed by primitive), 0: M = T (by template)
   float c_blend;
+  int exact_blend;       // 1: gradient of the blended colour w.r.t. logits and geometry;
+                         // 0: the reference's (no (1 - c_blend), no colour-map term)
   int N, Ht, Wt;
   int qmax;              // last valid texel-quad index (T*Ht*Wt - 1)
   float Wtm1, Htm1, hsx, hsy;
@@ -53,6 +55,25 @@ struct RasterArgs {
 __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   if ((nblocks & 7) != 0) return b;
   return (b & 7) * (nblocks >> 3) + (b >> 3);
+}
+
+template <int DT>
+using StoreT = typename std::conditional<DT == kF32, float,
+               typename std::conditional<DT == kF16, __half, __hip_bfloat16>::type>::type;
+
+// Blended colour and its derivative w.r.t. the texture coordinates (tx, ty).
+template <int DT>
+__device__ __forceinline__ void sample_color_grad(const RasterArgs& A, const PrimRec& p, int n,
+                                                  float tx, float ty, float col[3], float dcx[3],
+                                                  float dcy[3]) {
+  const long m = A.cmap_per_prim ? (long)n : (long)p.tmpl;
+  const StoreT<DT>* cm = (const StoreT<DT>*)A.cmap + m * (long)A.Ht * A.Wt * 3;
+  float co[3];
+  color_map_sample_grad(cm, A.Ht, A.Wt, tx, ty, co, dcx, dcy);
+  const float k = A.c_blend, k1 = 1.f - A.c_blend;
+  col[0] = k1 * p.cr + k * co[0];
+  col[1] = k1 * p.cg + k * co[1];
+  col[2] = k1 * p.cb + k * co[2];
 }
 
 template <int DT, bool BLEND>
@@ -321,11 +342,12 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
       const PrimRec p = srec[buf][j];
       const Sample<DT> S = eval_sample<DT, true>(A, p, fpx, fpy, kk < lst);
       if (!__any(S.pushed)) continue;
-      float col[3];
-      Geo geo;
-      geo.tx = S.tx;
-      geo.ty = S.ty;
-      sample_color<DT, BLEND>(A, p, sid[buf][j], geo, col);
+      float col[3], dcx[3] = {0.f, 0.f, 0.f}, dcy[3] = {0.f, 0.f, 0.f};
+      if constexpr (BLEND) {
+        sample_color_grad<DT>(A, p, sid[buf][j], S.tx, S.ty, col, dcx, dcy);
+      } else {
+        col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
+      }
       const float a = S.pushed ? p.abase * S.m : 0.f;
       const float Tk = T;
       const float wgt = Tk * a;
@@ -351,17 +373,28 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
       dLda = fmaf(gc1, fmaf(Tk, col[1], -S1 * inv1m), dLda);
       dLda = fmaf(gc2, fmaf(Tk, col[2], -S2 * inv1m), dLda);
       float acc[8];
-      // colour logits (tile_backward.cu:255-261); wgt = 0 for lanes not pushed
-      acc[5] = gc0 * wgt * (p.cr * (1.f - p.cr));
-      acc[6] = gc1 * wgt * (p.cg * (1.f - p.cg));
-      acc[7] = gc2 * wgt * (p.cb * (1.f - p.cb));
+      // colour logits (tile_backward.cu:255-261); wgt = 0 for lanes not pushed.
+      // With an exact colour-map gradient the blend weight (1 - c_blend)
+      // applies; the reference leaves it out.
+      const float cw = (BLEND && A.exact_blend) ? wgt * (1.f - A.c_blend) : wgt;
+      acc[5] = gc0 * cw * (p.cr * (1.f - p.cr));
+      acc[6] = gc1 * cw * (p.cg * (1.f - p.cg));
+      acc[7] = gc2 * cw * (p.cb * (1.f - p.cb));
       // opacity logit (:311-312): alpha_ub * mask * s (1 - s)
       const float me = S.pushed ? S.m : 0.f;
       acc[4] = dLda * (me * p.dav);
       // mask path (:314-343)
       const float dLdm = S.pushed ? dLda * p.abase : 0.f;
-      const float dLdu = dLdm * (S.gxt * A.hsx);
-      const float dLdv = dLdm * (S.gyt * A.hsy);
+      float dLdu = dLdm * (S.gxt * A.hsx);
+      float dLdv = dLdm * (S.gyt * A.hsy);
+      if constexpr (BLEND) {
+        if (A.exact_blend) {
+          // colour-map term: dC/dcol_k = T_k a_k, dcol/d(tx,ty) = c_blend dc_o/d(tx,ty)
+          const float kw = wgt * A.c_blend;
+          dLdu = fmaf(kw * A.hsx, fmaf(gc0, dcx[0], fmaf(gc1, dcx[1], gc2 * dcx[2])), dLdu);
+          dLdv = fmaf(kw * A.hsy, fmaf(gc0, dcy[0], fmaf(gc1, dcy[1], gc2 * dcy[2])), dLdv);
+        }
+      }
       const float pc = p.c * p.rinv, ps = p.s * p.rinv;
       acc[0] = fmaf(dLdu, -pc, dLdv * ps);
       acc[1] = fmaf(dLdu, -ps, -dLdv * pc);

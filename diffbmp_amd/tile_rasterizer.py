@@ -134,6 +134,7 @@ class _Run:
                 raise ValueError(f"colors_orig must be ({need},Ht,Wt,3), got {tuple(colors_orig.shape)}")
             self.colors_orig = cv(colors_orig)
         self.eps_T = float(eps_T)
+        self.blend_grad_exact = bool(getattr(handle, "blend_grad_exact", False))
         self.handle = handle
         if mapping is not None and mapping.numel() > 0:
             self.lists = _native.Lists(*unpack_mapping(mapping, handle.tiles))
@@ -146,7 +147,7 @@ class _Run:
 
     def bitmaps(self):
         return Handle.bitmaps(self.templates, self.colors_orig, self.co_mode, self.c_blend,
-                              self.tmpl_version)
+                              self.tmpl_version, self.blend_grad_exact)
 
     def forward(self):
         dev, H, W = self.means2D.device, self.handle.H, self.handle.W
@@ -311,6 +312,7 @@ class TileRasterizer:
         self.handle = Handle(image_height, image_width, tile_size, sigma, alpha_upper_bound,
                              max_prims_per_pixel, num_primitives)
         self.handle.eps_T = DEFAULT_EPS_T
+        self.handle.blend_grad_exact = False
         self.handle.store_dtype = (storage_dtype if storage_dtype is not None and use_fp16
                                    else (torch.float16 if use_fp16 else torch.float32))
         with _ACTIVE_LOCK:
@@ -344,6 +346,19 @@ class TileRasterizer:
     @eps_T.setter
     def eps_T(self, v: float) -> None:
         self.handle.eps_T = float(v)
+
+    @property
+    def exact_color_map_grad(self) -> bool:
+        """Extension, off by default.  Off: under ``c_blend > 0`` the colour
+        gradient is the reference's (``tile_backward.cu:253-258``), which leaves
+        out the ``(1 - c_blend)`` factor on the colour logits and the colour
+        map's dependence on position, radius and rotation.  On: the exact
+        gradient of the blended colour."""
+        return self.handle.blend_grad_exact
+
+    @exact_color_map_grad.setter
+    def exact_color_map_grad(self, v: bool) -> None:
+        self.handle.blend_grad_exact = bool(v)
 
     def enable_timing(self, on: bool = True) -> None:
         self.handle.timing_enable(on)

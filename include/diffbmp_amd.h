@@ -66,6 +66,9 @@ typedef struct {
   int32_t colors_orig_mode; /* 0: none, 1: per primitive (N,..), 2: per template (T,..) */
   float c_blend;            /* colour-map blend, used when > 0 and colors_orig set */
   int64_t version;          /* caller's content version of `templates`; -1 = unknown */
+  int32_t blend_grad_exact; /* 0: the reference's colour gradient under c_blend > 0
+                               (tile_backward.cu:253-258: no (1 - c_blend) factor, no
+                               dependence of the colour map on geometry); 1: exact */
 } dbmp_bitmaps;
 
 /* CSR tile lists: tile_offsets (tiles+1), tile_indices (P).  When the lists

@@ -48,6 +48,7 @@ class _Scene(ctypes.Structure):
         ("tile_offsets", ctypes.c_void_p), ("tile_indices", ctypes.c_void_p),
         ("geom64", ctypes.c_int),
         ("tile_subset", ctypes.c_void_p), ("n_subset", ctypes.c_int),
+        ("blend_grad_exact", ctypes.c_int), ("colors_orig_per_template", ctypes.c_int),
     ]
 
 
@@ -221,7 +222,8 @@ class OracleScene:
 
     def __init__(self, H, W, means2D, radii, rotations, opacities, colors, templates, bmp_sel,
                  tile_offsets, tile_indices, *, ts=16, alpha_ub=1.0, colors_orig=None,
-                 c_blend=0.0, geom64=False, tile_subset=None):
+                 c_blend=0.0, geom64=False, tile_subset=None, blend_grad_exact=False,
+                 colors_orig_per_template=False):
         c = lambda a, dt=f32: None if a is None else np.ascontiguousarray(a, dt)
         self.arrs = dict(
             means2D=c(means2D), radii=c(radii), rotations=c(rotations), opacities=c(opacities),
@@ -238,7 +240,9 @@ class OracleScene:
                         templates=_p(a["templates"]), bmp_sel=_p(a["bmp_sel"]),
                         tile_offsets=_p(a["tile_offsets"]), tile_indices=_p(a["tile_indices"]),
                         geom64=int(bool(geom64)), tile_subset=_p(a["tile_subset"]),
-                        n_subset=0 if tile_subset is None else len(a["tile_subset"]))
+                        n_subset=0 if tile_subset is None else len(a["tile_subset"]),
+                        blend_grad_exact=int(bool(blend_grad_exact)),
+                        colors_orig_per_template=int(bool(colors_orig_per_template)))
 
 
 def forward(sc: OracleScene, real: str = "f64"):
@@ -268,18 +272,22 @@ def count_samples(sc: OracleScene) -> int:
     return int(lib("f64").oracle_count_samples(ctypes.byref(sc.s)))
 
 
-def scene_from(scene, offs, idx, *, geom64=False, tile_subset=None, colors_orig=None, **over):
-    """OracleScene from a ``diffbmp_amd.scenes.Scene`` and tile lists."""
+def scene_from(scene, offs, idx, *, geom64=False, tile_subset=None, colors_orig=None,
+               blend_grad_exact=False, **over):
+    """OracleScene from a ``diffbmp_amd.scenes.Scene`` and tile lists.  The
+    scene's colour maps are passed per template (indexed by bmp_sel) unless
+    ``colors_orig`` gives them per primitive."""
     kw = dict(means2D=scene.means2D, radii=scene.radii, rotations=scene.rotations,
               opacities=scene.opacities, colors=scene.colors, templates=scene.templates)
     kw.update(over)
-    co = colors_orig
+    co, per_tmpl = colors_orig, False
     if co is None and scene.c_blend > 0 and scene.colors_orig_tmpl is not None:
-        co = scene.colors_orig_per_prim()
+        co, per_tmpl = scene.colors_orig_tmpl, True
     return OracleScene(scene.H, scene.W, kw["means2D"], kw["radii"], kw["rotations"], kw["opacities"],
                        kw["colors"], kw["templates"], scene.bmp_sel, offs, idx, ts=scene.tile_size,
                        alpha_ub=scene.alpha_ub, colors_orig=co, c_blend=scene.c_blend,
-                       geom64=geom64, tile_subset=tile_subset)
+                       geom64=geom64, tile_subset=tile_subset, blend_grad_exact=blend_grad_exact,
+                       colors_orig_per_template=per_tmpl)
 
 
 def bin_scene(scene):

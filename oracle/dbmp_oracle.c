@@ -56,7 +56,8 @@ typedef struct {
     const float *rotations;  /* (N)   */
     const float *opacities;  /* (N) logits */
     const float *colors;     /* (N,3) logits */
-    const float *colors_orig;/* (N,Ht,Wt,3) in [0,1] or NULL */
+    const float *colors_orig;/* (N,Ht,Wt,3), or (T,Ht,Wt,3) when colors_orig_per_template,
+                                in [0,1]; or NULL */
     float c_blend;
     const float *templates;  /* (T,Ht,Wt) */
     const int *bmp_sel;      /* (N) */
@@ -65,6 +66,10 @@ typedef struct {
     int geom64;              /* 0: fp32 decisions, 1: fp64 throughout */
     const int *tile_subset;  /* NULL: all tiles */
     int n_subset;
+    int blend_grad_exact;    /* 0: the reference's colour gradient under c_blend > 0
+                                (tile_backward.cu:253-258); 1: the exact gradient of
+                                the blended colour (logits and colour-map geometry) */
+    int colors_orig_per_template;
 } dbmp_oracle_scene;
 
 /* ------------------------------------------------------------------ */
@@ -91,6 +96,7 @@ typedef struct {
     REAL mask, gxt, gyt;      /* mask and its texel-space gradient */
     REAL u, v, ndx, ndy, c, s, rinv;
     REAL co[3];               /* colour-map sample (c_blend > 0) */
+    REAL dco_x[3], dco_y[3];  /* its derivative w.r.t. the texel coordinates */
 } sample_t;
 
 static REAL tex_at(const float *t, int Ht, int Wt, int yy, int xx) {
@@ -118,7 +124,8 @@ static int bilinear(const float *t, int Ht, int Wt, REAL tx, REAL ty,
 }
 
 /* sample_color_map, tile_forward.cu:17-89 (clamp-to-edge bilinear RGB) */
-static void color_map(const float *cm, int Ht, int Wt, REAL cx, REAL cy, REAL co[3]) {
+static void color_map(const float *cm, int Ht, int Wt, REAL cx, REAL cy, REAL co[3],
+                      REAL dcx[3], REAL dcy[3]) {
     if (cx < 0) cx = 0;
     if (cx > (REAL)(Wt - 1)) cx = (REAL)(Wt - 1);
     if (cy < 0) cy = 0;
@@ -133,6 +140,10 @@ static void color_map(const float *cm, int Ht, int Wt, REAL cx, REAL cy, REAL co
         REAL c0 = c00 * (1 - fx) + c10 * fx;
         REAL c1 = c01 * (1 - fx) + c11 * fx;
         co[ch] = c0 * (1 - fy) + c1 * fy;
+        /* derivative of the bilinear interpolant inside the cell; a clamped
+         * coordinate (x1 == x0) has zero derivative */
+        dcx[ch] = (c10 - c00) * (1 - fy) + (c11 - c01) * fy;
+        dcy[ch] = c1 - c0;
     }
 }
 
@@ -164,7 +175,8 @@ static int sample_prim(const dbmp_oracle_scene *s, int n, int px, int py, sample
         const float ftx = (u + 1.f) * 0.5f * (float)(Wt - 1);
         const float fty = (v + 1.f) * 0.5f * (float)(Ht - 1);
         if (s->colors_orig && s->c_blend > 0.f)
-            color_map(s->colors_orig + (size_t)n * Ht * Wt * 3, Ht, Wt, ftx, fty, o->co);
+            color_map(s->colors_orig + (size_t)(s->colors_orig_per_template ? ti : n) * Ht * Wt * 3, Ht, Wt, ftx, fty, o->co,
+                      o->dco_x, o->dco_y);
         /* NaN coordinates are rejected too (undefined in the reference) */
         if (!(ftx >= 0.f && ftx <= (float)(Wt - 1) && fty >= 0.f && fty <= (float)(Ht - 1))) return 0;
         x0 = (int)floorf(ftx);
@@ -191,7 +203,8 @@ static int sample_prim(const dbmp_oracle_scene *s, int n, int px, int py, sample
         const double dtx = (u + 1.0) * 0.5 * (Wt - 1);
         const double dty = (v + 1.0) * 0.5 * (Ht - 1);
         if (s->colors_orig && s->c_blend > 0.f)
-            color_map(s->colors_orig + (size_t)n * Ht * Wt * 3, Ht, Wt, (REAL)dtx, (REAL)dty, o->co);
+            color_map(s->colors_orig + (size_t)(s->colors_orig_per_template ? ti : n) * Ht * Wt * 3, Ht, Wt, (REAL)dtx, (REAL)dty, o->co,
+                      o->dco_x, o->dco_y);
         if (!(dtx >= 0 && dtx <= (double)(Wt - 1) && dty >= 0 && dty <= (double)(Ht - 1))) return 0;
         x0 = (int)floor(dtx);
         y0 = (int)floor(dty);
@@ -453,10 +466,14 @@ int oracle_backward(const dbmp_oracle_scene *s, const REAL *gCin, const REAL *gA
                         const sample_t *sm = &pc.smp[j];
                         REAL *g = acc + (size_t)8 * n;
                         /* (1) colour logits, tile_backward.cu:228-262 (no
-                         * (1-c_blend) factor, as in the reference) */
+                         * (1-c_blend) factor in the reference; the exact
+                         * gradient has it) */
+                        const int blend = s->colors_orig && s->c_blend > 0.f;
+                        const int exact = blend && s->blend_grad_exact;
+                        const REAL cw = exact ? (REAL)1 - (REAL)s->c_blend : (REAL)1;
                         for (int ch = 0; ch < 3; ++ch) {
                             const REAL si = sigm((REAL)s->colors[3 * n + ch]);
-                            g[5 + ch] += gC[ch] * (a * Tk) * (si * ((REAL)1 - si));
+                            g[5 + ch] += gC[ch] * (a * Tk) * cw * (si * ((REAL)1 - si));
                         }
                         /* (2) dL/dalpha, :264-271 */
                         REAL om = (REAL)1 - a;
@@ -469,8 +486,19 @@ int oracle_backward(const dbmp_oracle_scene *s, const REAL *gCin, const REAL *gA
                         const REAL sop = sigm((REAL)s->opacities[n]);
                         g[4] += dLda * ((REAL)s->alpha_ub * sm->mask * sop * ((REAL)1 - sop));
                         const REAL dLdm = dLda * ((REAL)s->alpha_ub * sop);
-                        const REAL dLdu = dLdm * (sm->gxt * hsx);
-                        const REAL dLdv = dLdm * (sm->gyt * hsy);
+                        REAL dLdu = dLdm * (sm->gxt * hsx);
+                        REAL dLdv = dLdm * (sm->gyt * hsy);
+                        if (exact) {
+                            /* C = sum_k T_k a_k col_k, col = (1-k) sig + k c_o(tx, ty) */
+                            REAL gx = 0, gy = 0;
+                            for (int ch = 0; ch < 3; ++ch) {
+                                gx += gC[ch] * sm->dco_x[ch];
+                                gy += gC[ch] * sm->dco_y[ch];
+                            }
+                            const REAL kw = a * Tk * (REAL)s->c_blend;
+                            dLdu += kw * gx * hsx;
+                            dLdv += kw * gy * hsy;
+                        }
                         const REAL c = sm->c, sn = sm->s, ri = sm->rinv;
                         g[0] += dLdu * (-c * ri) + dLdv * (sn * ri);
                         g[1] += dLdu * (-sn * ri) + dLdv * (-c * ri);

@@ -31,18 +31,23 @@ def rounded(scene, dtype):
 class GpuRun:
     """Bins, renders and back-propagates one scene through the C ABI."""
 
-    def __init__(self, scene, dev, dtype=torch.float32, eps_T=0.0, ts=16, co_mode=0):
+    def __init__(self, scene, dev, dtype=torch.float32, eps_T=0.0, ts=16, co_mode=0,
+                 bm_dtype=None, exact_blend=False):
+        """``dtype``: parameter storage; ``bm_dtype``: bitmap and colour-map
+        storage (default ``dtype``)."""
         self.scene, self.dev, self.dtype, self.eps = scene, dev, dtype, eps_T
+        bm_dtype = dtype if bm_dtype is None else bm_dtype
         self.h = Handle(scene.H, scene.W, ts, 0.0, scene.alpha_ub, 151, scene.N)
         self.d = to_dev(scene, dev, dtype)
         d = self.d
+        d["templates"] = d["templates"].to(bm_dtype)
         self.prims = Handle.prims(d["means2D"], d["radii"], d["rotations"], d["opacities"],
                                   d["colors"], d["bmp_sel"])
         self.co = None
         if co_mode:
             src = scene.colors_orig_tmpl if co_mode == 2 else scene.colors_orig_per_prim()
-            self.co = torch.from_numpy(np.ascontiguousarray(src)).to(dev).to(dtype)
-        self.bm = Handle.bitmaps(d["templates"], self.co, co_mode, scene.c_blend, -1)
+            self.co = torch.from_numpy(np.ascontiguousarray(src)).to(dev).to(bm_dtype)
+        self.bm = Handle.bitmaps(d["templates"], self.co, co_mode, scene.c_blend, -1, exact_blend)
         self.ext = self.h.template_extents(d["templates"])
         self.offs, self.idx = self.h.bin(self.prims, self.ext, tuple(d["templates"].shape))
 
@@ -84,3 +89,54 @@ def rand_grads(H, W, seed=0, scale=1.0):
     rng = np.random.default_rng(seed)
     return (rng.standard_normal((H, W, 3)) * scale).astype(np.float32), \
            (rng.standard_normal((H, W)) * scale).astype(np.float32)
+
+
+def half_rounded(a):
+    """fp32 array rounded to fp16 storage and back."""
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).half().float().numpy()
+
+
+def fd_window(templates, support=0.95):
+    """Templates times the C2 window ``(1 - (rho/R)^2)^3`` (zero beyond R =
+    ``support`` of the half size), so that every bitmap is exactly zero over
+    its outer texels and the push box never cuts a nonzero mask."""
+    T, Ht, Wt = templates.shape
+    yy, xx = np.mgrid[0:Ht, 0:Wt].astype(np.float64)
+    hy, hx = (Ht - 1) / 2, (Wt - 1) / 2
+    rho2 = ((xx - hx) / (hx * support)) ** 2 + ((yy - hy) / (hy * support)) ** 2
+    win = np.where(rho2 < 1.0, (1.0 - rho2) ** 3, 0.0)
+    return (templates * win[None]).astype(np.float32)
+
+
+def fd_scalars(scene, offs, idx, g, gC, gA, picks, h, **oracle_kw):
+    """Central differences of L = sum(gC * C) + sum(gA * A) (fp64 oracle,
+    geometry in fp64, tile lists frozen) for each (key, n, j) in ``picks``.
+    L is summed over the tiles whose lists hold primitive n, the only pixels
+    its parameters reach.  The step is max(h, 2 ulp) of the fp32 parameter
+    and the step actually stored is used.  -> (fd, analytic) arrays."""
+    tiles = len(offs) - 1
+    tX = -(-scene.W // scene.tile_size)
+    tile_of = np.repeat(np.arange(tiles), np.diff(offs))
+    ts = scene.tile_size
+    fd, an = [], []
+    for key, n, j in picks:
+        sub = np.ascontiguousarray(tile_of[idx == n], np.int32)
+        m = np.zeros(tiles, bool)
+        m[sub] = True
+        pm = np.kron(m.reshape(-1, tX), np.ones((ts, ts), bool))[:scene.H, :scene.W]
+        arr = getattr(scene, key)
+        base = arr.copy()
+        ix = (n,) if arr.ndim == 1 else (n, j)
+        step = max(h, 2.0 * float(np.spacing(np.float32(base[ix]))))
+        vals = []
+        for sgn in (1.0, -1.0):
+            a2 = base.copy()
+            a2[ix] = base[ix] + sgn * step
+            setattr(scene, key, a2)
+            osc = O.scene_from(scene, offs, idx, geom64=True, tile_subset=sub, **oracle_kw)
+            col, al, _ = O.forward(osc)
+            vals.append(((col * gC)[pm].sum() + (al * gA)[pm].sum(), float(np.float32(a2[ix]))))
+        setattr(scene, key, base)
+        fd.append((vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1]))
+        an.append(float(g[key][ix]))
+    return np.array(fd), np.array(an)

@@ -272,6 +272,76 @@ def test_backward_colour_logit_closed_form():
         assert abs(g["colors"][0, ch] - gC[8, 8, ch] * al * s * (1 - s)) < 1e-14
 
 
+@pytest.fixture(scope="module")
+def blend_case():
+    """A colour-mapped scene (c_blend = 0.6, low-frequency per-primitive maps)."""
+    p = _fd_scene(21)
+    rng = np.random.default_rng(22)
+    N = len(p["r"])
+    p["co"] = np.stack([S.smooth_field(32, 32, rng, terms=4) for _ in range(N)])
+    gC = rng.standard_normal((p["H"], p["W"], 3))
+    gA = rng.standard_normal((p["H"], p["W"]))
+    return p, gC, gA
+
+
+def _blend_scene(p, exact):
+    return _scene(p["H"], p["W"], p["m"], p["r"], p["th"], p["v"], p["c"], p["tm"], sel=p["sel"],
+                  geom64=True, colors_orig=p["co"], c_blend=0.6, blend_grad_exact=exact)
+
+
+def test_backward_fd_exact_colour_map(blend_case):
+    """With the exact colour-map gradient, every parameter's derivative matches
+    central differences (h = 1e-5, fp64, 1e-3 relative, floor 1e-4 of the
+    largest) through the blended colour (1 - k) sig(c) + k c_o(tx, ty)."""
+    p, gC, gA = blend_case
+    g = O.backward(_blend_scene(p, True), gC, gA)
+    rng = np.random.default_rng(23)
+    N = len(p["r"])
+    keys = [("m", "means2D"), ("r", "radii"), ("th", "rotations"), ("v", "opacities"), ("c", "colors")]
+    picks = [keys[i % 5] + (int(rng.integers(N)), int(rng.integers(3))) for i in range(40)]
+
+    def loss(q):
+        col, a, _ = O.forward(_blend_scene(q, True))
+        return float((col * gC).sum() + (a * gA).sum())
+
+    fd, an = [], []
+    h = 1e-5
+    for pk, gk, n, j in picks:
+        arr = p[pk]
+        ix = (n,) if arr.ndim == 1 else (n, j % arr.shape[1])
+        base = arr.copy()
+        plus, minus = base.copy(), base.copy()
+        plus[ix] += h
+        minus[ix] -= h
+        p[pk] = plus
+        lp = loss(p)
+        p[pk] = minus
+        lm = loss(p)
+        p[pk] = base
+        h_eff = (float(np.float32(plus[ix])) - float(np.float32(minus[ix]))) / 2
+        fd.append((lp - lm) / (2 * h_eff))
+        an.append(float(g[gk][ix]))
+    fd, an = np.array(fd), np.array(an)
+    tol = 1e-3 * np.abs(an) + 1e-4 * np.abs(an).max()
+    bad = np.abs(fd - an) > tol
+    assert not bad.any(), [(picks[i][1], fd[i], an[i]) for i in np.nonzero(bad)[0]]
+
+
+def test_backward_reference_colour_gradient_under_blend(blend_case):
+    """The reference's gradient under c_blend > 0 drops the (1 - k) factor on
+    the colour logits and the colour map's dependence on geometry: its colour
+    gradient is exactly the exact one / (1 - k), and its geometry gradient
+    differs from the exact one."""
+    p, gC, gA = blend_case
+    ge = O.backward(_blend_scene(p, True), gC, gA)
+    gr = O.backward(_blend_scene(p, False), gC, gA)
+    k1 = 1.0 - float(np.float32(0.6))
+    assert np.allclose(gr["colors"] * k1, ge["colors"], rtol=1e-12, atol=1e-15)
+    assert np.allclose(gr["opacities"], ge["opacities"], rtol=1e-12, atol=1e-15)
+    d = np.linalg.norm(gr["means2D"] - ge["means2D"]) / np.linalg.norm(ge["means2D"])
+    assert d > 1e-3, d
+
+
 # ---------------------------------------------------------------- fp32 port
 def test_fp32_port_agrees_with_fp64_oracle():
     sc = S.make_scene(0, H=96, W=96, N=64, with_target=False)
