@@ -58,6 +58,25 @@ class TileLists(ctypes.Structure):
                 ("pair_count", ctypes.c_void_p), ("capacity", ctypes.c_int64)]
 
 
+class Job(ctypes.Structure):
+    """``dbmp_job``: one fit job of a batched step."""
+    _fields_ = [("height", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("num_primitives", ctypes.c_int32), ("num_templates", ctypes.c_int32),
+                ("tmpl_height", ctypes.c_int32), ("tmpl_width", ctypes.c_int32),
+                ("quads", ctypes.c_void_p), ("ext", ctypes.c_void_p),
+                ("colors_orig", ctypes.c_void_p), ("c_blend", ctypes.c_float),
+                ("blend_grad_exact", ctypes.c_int32), ("bmp_sel", ctypes.c_void_p),
+                ("params", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
+                ("exp_avg_sq", ctypes.c_void_p), ("store", ctypes.c_void_p),
+                ("target", ctypes.c_void_p), ("background", ctypes.c_void_p),
+                ("background_value", ctypes.c_float), ("out_color", ctypes.c_void_p),
+                ("out_alpha", ctypes.c_void_p), ("out_T", ctypes.c_void_p),
+                ("out_last", ctypes.c_void_p), ("loss_history", ctypes.c_void_p),
+                ("loss_capacity", ctypes.c_int32), ("lr", ctypes.c_float * 5),
+                ("alpha_upper_bound", ctypes.c_float), ("eps_T", ctypes.c_float),
+                ("clamp", ctypes.c_int32), ("steps_done", ctypes.c_int32)]
+
+
 class Lists:
     """Tile lists on the device: CSR offsets, index buffer, and -- for lists
     produced by dbmp_bin -- the device pair count and the capacity used."""
@@ -108,6 +127,16 @@ EXPORTS = {
                                          ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_int64)]),
     "dbmp_timing_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "dbmp_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
+                                         ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "dbmp_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "dbmp_batch_prepare_bitmaps": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Bitmaps),
+                                                  ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
+    "dbmp_batch_set_jobs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Job), ctypes.c_int32,
+                                           ctypes.c_int64, ctypes.c_void_p]),
+    "dbmp_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "dbmp_batch_pair_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
 }
 
 

@@ -5,7 +5,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "capi.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("capi.hip", "batch.hip")]
 OUT = os.path.join(HERE, "libdiffbmp_amd.so")
 
 HIPCC_FLAGS = [
@@ -29,7 +29,7 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> str:
     """Compile the HIP sources into ``diffbmp_amd/libdiffbmp_amd.so``."""
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SRCS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)

@@ -24,7 +24,7 @@ constexpr int kRadixTile = kRadixThreads * kRadixItems;  // keys per block
 // Template extents (vector_renderer.py:151-192): min/max column/row of texels
 // > 1e-6 mapped through u = (col/(W-1))*2 - 1 in fp32; empty => (-1,1,-1,1).
 template <typename TT>
-__global__ __launch_bounds__(256) void k_template_extents(const TT* __restrict__ tmpl, int Ht,
+static __global__ __launch_bounds__(256) void k_template_extents(const TT* __restrict__ tmpl, int Ht,
                                                           int Wt, float* __restrict__ ext) {
   const int t = blockIdx.x;
   const TT* m = tmpl + (long)t * Ht * Wt;
@@ -91,23 +91,15 @@ __device__ __forceinline__ void tile_interval(float bmin, float bmax, int nt, in
   }
 }
 
-template <typename PT>
-__global__ __launch_bounds__(256) void k_prep(
-    int N, const PT* __restrict__ means2D, const PT* __restrict__ radii,
-    const PT* __restrict__ rotations, const PT* __restrict__ opacities,
-    const PT* __restrict__ colors, const int* __restrict__ sel, const float* __restrict__ ext,
-    int T, int Ht, int Wt, int H, int W, int ts, int tilesX, int tilesY, float alpha_ub,
-    PrimRec* __restrict__ recs, int* __restrict__ counts, int4* __restrict__ ranges) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  const float x = ldf(means2D, 2L * n), y = ldf(means2D, 2L * n + 1);
-  const float r = ldf(radii, n), th = ldf(rotations, n);
-  const float v = ldf(opacities, n);
-  const int t = sel[n];
+// Raster record and tile range of one primitive (parameters already in fp32).
+// ext == nullptr: raster constants only (cnt = 0).
+__device__ __forceinline__ void prep_core(float x, float y, float r, float th, float v, float c0,
+                                          float c1, float c2, int t, const float* __restrict__ ext,
+                                          int T, int Ht, int Wt, int H, int W, int ts, int tilesX,
+                                          int tilesY, float alpha_ub, PrimRec& p, int& cnt,
+                                          int4& rg) {
   // correctly rounded fp32 trig, shared by binning and raster
   const float c = (float)cos((double)th), s = (float)sin((double)th);
-
-  PrimRec p;
   p.mx = x; p.my = y; p.r = r; p.c = c; p.s = s;
   const float tw = (float)Wt, tht = (float)Ht;
   const float md = fmaxf(tw, tht);
@@ -117,9 +109,9 @@ __global__ __launch_bounds__(256) void k_prep(
   p.rinv = r > 1e-6f ? 1.f / r : 1e6f;
   p.sop = sigmoid_safe(v);
   p.abase = alpha_ub * p.sop;
-  p.cr = sigmoid_safe(ldf(colors, 3L * n));
-  p.cg = sigmoid_safe(ldf(colors, 3L * n + 1));
-  p.cb = sigmoid_safe(ldf(colors, 3L * n + 2));
+  p.cr = sigmoid_safe(c0);
+  p.cg = sigmoid_safe(c1);
+  p.cb = sigmoid_safe(c2);
   p.dav = alpha_ub * p.sop * (1.f - p.sop);
   const bool valid = (t >= 0 && t < T);
   p.tmpl = valid ? t : -1;
@@ -128,11 +120,9 @@ __global__ __launch_bounds__(256) void k_prep(
     p.hw = -1.f;
     p.hh = -1.f;
   }
-  recs[n] = p;
-
-  int cnt = 0;
-  int4 rg = make_int4(0, -1, 0, -1);
-  if (valid && ext != nullptr) {   // ext == nullptr: raster constants only
+  cnt = 0;
+  rg = make_int4(0, -1, 0, -1);
+  if (valid && ext != nullptr) {
     const float* e = ext + 4 * t;
     const float cu[4] = {e[0], e[1], e[0], e[1]};
     const float cv[4] = {e[2], e[2], e[3], e[3]};
@@ -159,6 +149,25 @@ __global__ __launch_bounds__(256) void k_prep(
       rg = make_int4(tx0, tx1, ty0, ty1);
     }
   }
+}
+
+template <typename PT>
+static __global__ __launch_bounds__(256) void k_prep(
+    int N, const PT* __restrict__ means2D, const PT* __restrict__ radii,
+    const PT* __restrict__ rotations, const PT* __restrict__ opacities,
+    const PT* __restrict__ colors, const int* __restrict__ sel, const float* __restrict__ ext,
+    int T, int Ht, int Wt, int H, int W, int ts, int tilesX, int tilesY, float alpha_ub,
+    PrimRec* __restrict__ recs, int* __restrict__ counts, int4* __restrict__ ranges) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  PrimRec p;
+  int cnt;
+  int4 rg;
+  prep_core(ldf(means2D, 2L * n), ldf(means2D, 2L * n + 1), ldf(radii, n), ldf(rotations, n),
+            ldf(opacities, n), ldf(colors, 3L * n), ldf(colors, 3L * n + 1),
+            ldf(colors, 3L * n + 2), sel[n], ext, T, Ht, Wt, H, W, ts, tilesX, tilesY, alpha_ub,
+            p, cnt, rg);
+  recs[n] = p;
   counts[n] = cnt;
   ranges[n] = rg;
 }
@@ -193,7 +202,7 @@ __device__ __forceinline__ long long block256_excl_scan(long long v, long long& 
 }
 
 // offs[i] = sum(counts[0..i)), offs[N] = *total = sum(counts).
-__global__ __launch_bounds__(256) void k_scan_counts(const int* __restrict__ counts, int N,
+static __global__ __launch_bounds__(256) void k_scan_counts(const int* __restrict__ counts, int N,
                                                      long long* __restrict__ offs,
                                                      long long* __restrict__ total) {
   __shared__ long long sh[4];
@@ -224,7 +233,7 @@ __global__ __launch_bounds__(256) void k_scan_counts(const int* __restrict__ cou
 }
 
 // One wave per primitive; pairs land at offs[n] in row-major tile order.
-__global__ __launch_bounds__(256) void k_emit(int N, const int* __restrict__ counts,
+static __global__ __launch_bounds__(256) void k_emit(int N, const int* __restrict__ counts,
                                               const long long* __restrict__ offs,
                                               const int4* __restrict__ ranges, int tilesX,
                                               const long long* __restrict__ total, long long cap,
@@ -250,7 +259,7 @@ __device__ __forceinline__ long long pairs_in_range(const long long* total, long
 }
 
 // Per-block digit histogram, stored digit-major: hist[d * nblk + b].
-__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(
+static __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(
     const int* __restrict__ keys, const long long* __restrict__ total, long long cap, int shift,
     int bits, int nblk, int* __restrict__ hist) {
   __shared__ int h[256];
@@ -269,7 +278,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(
 }
 
 // One block per digit: exclusive scan of that digit's row over blocks.
-__global__ __launch_bounds__(256) void k_radix_rowscan(int* __restrict__ hist, int nblk,
+static __global__ __launch_bounds__(256) void k_radix_rowscan(int* __restrict__ hist, int nblk,
                                                        int* __restrict__ totals) {
   __shared__ long long sh[4];
   int* row = hist + (long)blockIdx.x * nblk;
@@ -290,7 +299,7 @@ __global__ __launch_bounds__(256) void k_radix_rowscan(int* __restrict__ hist, i
 // items by digit with ballots and a per-wave running count in LDS.  The block
 // then reorders its keys by digit in LDS and writes each digit's run
 // contiguously, so global writes are coalesced.
-__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
+static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     const int* __restrict__ kin, const int* __restrict__ vin, int* __restrict__ kout,
     int* __restrict__ vout, const long long* __restrict__ total, long long cap, int shift,
     int bits, int nblk, const int* __restrict__ hist, const int* __restrict__ totals) {
@@ -386,7 +395,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
 // tile_offsets[t] = first index whose tile id >= t, tile_offsets[tiles] = P,
 // from the boundaries of the sorted ids.  On overflow (P > cap) every offset
 // is 0.  Needs max(cap, tiles) + 1 threads.
-__global__ __launch_bounds__(256) void k_tile_bounds(const int* __restrict__ keys,
+static __global__ __launch_bounds__(256) void k_tile_bounds(const int* __restrict__ keys,
                                                      const long long* __restrict__ total,
                                                      long long cap, int tiles,
                                                      int* __restrict__ tile_offsets) {

@@ -17,69 +17,14 @@
 
 #include "../../include/diffbmp_amd.h"
 #include "bin_kernels.h"
+#include "host_common.h"
 #include "raster_kernels.h"
 #include "step_kernels.h"
 
 using namespace dbmp;
+using namespace dbmp_host;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                               \
-  do {                                                                              \
-    hipError_t e_ = (expr);                                                         \
-    if (e_ != hipSuccess)                                                           \
-      return fail(DBMP_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
-  } while (0)
-
-#define LAUNCH_CHECK(name)                                                          \
-  do {                                                                              \
-    hipError_t e_ = hipGetLastError();                                              \
-    if (e_ != hipSuccess)                                                           \
-      return fail(DBMP_E_HIP, "launch of %s failed: %s", name, hipGetErrorString(e_)); \
-  } while (0)
-
-int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
-
-__global__ void k_fill_f32(float* p, long long n, float v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-template <int DT>
-__global__ void k_build_quads(const void* src, int T, int Ht, int Wt, void* dst) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long tot = (long long)T * Ht * Wt;
-  if (i >= tot) return;
-  const int x = (int)(i % Wt);
-  const int y = (int)((i / Wt) % Ht);
-  const bool rx = x + 1 < Wt, ry = y + 1 < Ht;
-  if constexpr (DT == kF32) {
-    const float* s = (const float*)src;
-    float4 q;
-    q.x = s[i];
-    q.y = rx ? s[i + 1] : 0.f;
-    q.z = ry ? s[i + Wt] : 0.f;
-    q.w = (rx && ry) ? s[i + Wt + 1] : 0.f;
-    ((float4*)dst)[i] = q;
-  } else {
-    const unsigned short* s = (const unsigned short*)src;   // raw fp16 / bf16 bits
-    const uint32_t a = s[i], b = rx ? s[i + 1] : 0u, c = ry ? s[i + Wt] : 0u,
-                   d = (rx && ry) ? s[i + Wt + 1] : 0u;
-    ((uint2*)dst)[i] = make_uint2(a | (b << 16), c | (d << 16));
-  }
-}
 
 __global__ void k_unpack_grads(const float* __restrict__ g8, int N, float* __restrict__ gm,
                                float* __restrict__ gr, float* __restrict__ gt,
@@ -134,20 +79,6 @@ struct dbmp_rasterizer {
 };
 
 namespace {
-
-template <typename T>
-int grow(T** p, long long n) {
-  if (*p) (void)hipFree(*p);
-  *p = nullptr;
-  if (n <= 0) return 0;
-  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(DBMP_E_ALLOC, "hipMalloc of %lld bytes failed: %s", (long long)(sizeof(T) * n),
-                hipGetErrorString(e));
-  }
-  return 0;
-}
 
 int ensure_n(dbmp_rasterizer* h, int N) {
   if (N <= h->n_cap) return 0;
@@ -238,14 +169,8 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
     if (e != hipSuccess) return fail(DBMP_E_ALLOC, "hipMalloc quads: %s", hipGetErrorString(e));
     h->quads_bytes = (size_t)tot * esz;
   }
-  const dim3 grid(ceil_div(tot, 256)), block(256);
-  if (b->dtype == DBMP_F32)
-    hipLaunchKernelGGL(k_build_quads<kF32>, grid, block, 0, st, b->templates, b->num_templates,
-                       b->height, b->width, h->quads);
-  else
-    hipLaunchKernelGGL(k_build_quads<kF16>, grid, block, 0, st, b->templates, b->num_templates,
-                       b->height, b->width, h->quads);
-  LAUNCH_CHECK("k_build_quads");
+  int rc = launch_build_quads(b, h->quads, st);
+  if (rc) return rc;
   h->q_src = b->templates;
   h->q_version = b->version;
   h->q_T = b->num_templates;

@@ -1,0 +1,106 @@
+// Host-side helpers shared by the C ABI translation units: error state,
+// checked HIP calls, device allocation, fill and texel-quad kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/diffbmp_amd.h"
+#include "dbmp_device.h"
+
+namespace dbmp_host {
+
+using dbmp::kF32;
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(DBMP_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                                          \
+  do {                                                                              \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess)                                                           \
+      return fail(DBMP_E_HIP, "launch of %s failed: %s", name, hipGetErrorString(e_)); \
+  } while (0)
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+static __global__ void k_fill_f32(float* p, long long n, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <int DT>
+static __global__ void k_build_quads(const void* src, int T, int Ht, int Wt, void* dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long tot = (long long)T * Ht * Wt;
+  if (i >= tot) return;
+  const int x = (int)(i % Wt);
+  const int y = (int)((i / Wt) % Ht);
+  const bool rx = x + 1 < Wt, ry = y + 1 < Ht;
+  if constexpr (DT == kF32) {
+    const float* s = (const float*)src;
+    float4 q;
+    q.x = s[i];
+    q.y = rx ? s[i + 1] : 0.f;
+    q.z = ry ? s[i + Wt] : 0.f;
+    q.w = (rx && ry) ? s[i + Wt + 1] : 0.f;
+    ((float4*)dst)[i] = q;
+  } else {
+    const unsigned short* s = (const unsigned short*)src;   // raw fp16 / bf16 bits
+    const uint32_t a = s[i], b = rx ? s[i + 1] : 0u, c = ry ? s[i + Wt] : 0u,
+                   d = (rx && ry) ? s[i + Wt + 1] : 0u;
+    ((uint2*)dst)[i] = make_uint2(a | (b << 16), c | (d << 16));
+  }
+}
+
+template <typename T>
+inline int grow(T** p, long long n) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  if (n <= 0) return 0;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(DBMP_E_ALLOC, "hipMalloc of %lld bytes failed: %s", (long long)(sizeof(T) * n),
+                hipGetErrorString(e));
+  }
+  return 0;
+}
+
+
+// Texel quads of a template set: entry i holds the 2x2 neighbourhood
+// (i, i+1, i+W, i+W+1), zero outside the bitmap; 16 bytes (fp32) or 8 bytes
+// (fp16 / bf16 bit patterns) each.
+inline int launch_build_quads(const dbmp_bitmaps* b, void* dst, hipStream_t st) {
+  const long long tot = (long long)b->num_templates * b->height * b->width;
+  if (tot == 0) return 0;
+  const dim3 grid(ceil_div(tot, 256)), block(256);
+  if (b->dtype == DBMP_F32)
+    hipLaunchKernelGGL(k_build_quads<dbmp::kF32>, grid, block, 0, st, b->templates,
+                       b->num_templates, b->height, b->width, dst);
+  else
+    hipLaunchKernelGGL(k_build_quads<dbmp::kF16>, grid, block, 0, st, b->templates,
+                       b->num_templates, b->height, b->width, dst);
+  LAUNCH_CHECK("k_build_quads");
+  return 0;
+}
+
+}  // namespace dbmp_host

@@ -80,7 +80,9 @@ template <int DT, bool BLEND>
 __device__ __forceinline__ void sample_color(const RasterArgs& A, const PrimRec& p, int n,
                                              const Geo& g, float col[3]) {
   col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
-  if constexpr (BLEND) {
+  // a batch mixing canvases with and without colour maps runs BLEND kernels;
+  // a canvas without one keeps its plain colours
+  if (BLEND && A.cmap != nullptr) {
     using CT = typename std::conditional<DT == kF32, float,
                typename std::conditional<DT == kF16, __half, __hip_bfloat16>::type>::type;
     const long m = A.cmap_per_prim ? (long)n : (long)p.tmpl;
@@ -100,9 +102,9 @@ struct BlockGeom {
   int tile, bx, by, px, py, w, lane;
 };
 
-__device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A) {
+// Geometry of workgroup b (already remapped) of a canvas.
+__device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A, int b) {
   BlockGeom g;
-  const int b = xcd_remap(blockIdx.x, A.tiles * A.subs);
   g.tile = b / A.subs;
   const int sub = b - g.tile * A.subs;
   const int spr = A.ts / kSub;
@@ -220,15 +222,17 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   return r;
 }
 
+// Forward of one 16x16 block b of a canvas; the caller's workgroup runs it.
 template <int DT, bool BLEND>
-__global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
-                                                         float* __restrict__ out_alpha,
-                                                         float* __restrict__ out_T,
-                                                         int* __restrict__ out_last) {
+__device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
+                                                 float* __restrict__ out_color,
+                                                 float* __restrict__ out_alpha,
+                                                 float* __restrict__ out_T,
+                                                 int* __restrict__ out_last) {
   __shared__ PrimRec srec[2][kBatch];
   __shared__ int sid[2][kBatch];
   __shared__ int smask[2][kBatch];
-  const BlockGeom G = block_geom(A);
+  const BlockGeom G = block_geom(A, b);
   const float fpx = (float)G.px, fpy = (float)G.py;
   bool alive = (G.px < A.W) && (G.py < A.H);
   // T: transmittance; Tnz: product of the nonzero (1 - a) factors; nz: count
@@ -289,33 +293,81 @@ __global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __
   }
 }
 
-// grad8[n*8 + i]: 0 mx, 1 my, 2 r, 3 theta, 4 opacity logit, 5..7 colour logits
 template <int DT, bool BLEND>
-__global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const float* __restrict__ gCin,
-                                                         const float* __restrict__ gAin,
-                                                         const float* __restrict__ Cfin,
-                                                         const float* __restrict__ Tfin,
-                                                         const int* __restrict__ lastin,
-                                                         float* __restrict__ grad8) {
+__global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
+                                                         float* __restrict__ out_alpha,
+                                                         float* __restrict__ out_T,
+                                                         int* __restrict__ out_last) {
+  raster_fwd_block<DT, BLEND>(A, xcd_remap(blockIdx.x, A.tiles * A.subs), out_color, out_alpha,
+                              out_T, out_last);
+}
+
+// Background composite + MSE, fused in front of the backward:
+//   result = C + (1 - A) bg,  r = result - target,  loss += sum(r^2) / (3 H W)
+//   dL/dC = 2 r / (3 H W),    dL/dA = -sum_c dL/dC_c bg_c
+// (the same arithmetic as k_loss_mse).
+struct LossArgs {
+  const float* target;   // (H,W,3)
+  const float* bg;       // (H,W,3) or null for the constant bg_value
+  float bg_value;
+  float inv_count;       // 1 / (3 H W)
+  float* loss;           // accumulated (+=)
+};
+
+// grad8[n*8 + i]: 0 mx, 1 my, 2 r, 3 theta, 4 opacity logit, 5..7 colour logits
+// Backward of one 16x16 block b.  With LOSS the pixel gradients come from the
+// fused background composite + MSE (L; gCin / gAin unused), else from gCin /
+// gAin (L unused).
+template <int DT, bool BLEND, bool LOSS>
+__device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
+                                                 const float* __restrict__ gCin,
+                                                 const float* __restrict__ gAin,
+                                                 const LossArgs& L,
+                                                 const float* __restrict__ Cfin,
+                                                 const float* __restrict__ Tfin,
+                                                 const int* __restrict__ lastin,
+                                                 float* __restrict__ grad8) {
   __shared__ PrimRec srec[2][kBatch];
   __shared__ int sid[2][kBatch];
   __shared__ int smask[2][kBatch];
   __shared__ float sred[4][kBatch][8];
   __shared__ int skmax;
-  const BlockGeom G = block_geom(A);
+  const BlockGeom G = block_geom(A, b);
   const float fpx = (float)G.px, fpy = (float)G.py;
   const bool okpix = (G.px < A.W) && (G.py < A.H);
   const long pix = okpix ? (long)G.py * A.W + G.px : 0;
   const int lstw = okpix ? lastin[pix] : 0;
   const int lst = lstw & 0x3fffffff;   // list entries consumed by the forward
   const int nzf = (unsigned)lstw >> 30; // exactly opaque layers seen (capped at 2)
-  const float gc0 = okpix ? gCin[3 * pix] : 0.f, gc1 = okpix ? gCin[3 * pix + 1] : 0.f,
-              gc2 = okpix ? gCin[3 * pix + 2] : 0.f;
-  const float ga = (okpix && gAin) ? gAin[pix] : 0.f;
   const float cf0 = okpix ? Cfin[3 * pix] : 0.f, cf1 = okpix ? Cfin[3 * pix + 1] : 0.f,
               cf2 = okpix ? Cfin[3 * pix + 2] : 0.f;
   const float tnzf = okpix ? Tfin[pix] : 0.f;   // product of the nonzero (1 - a) factors
   const float tf = nzf ? 0.f : tnzf;            // transmittance after the last layer
+  float gc0, gc1, gc2, ga;
+  if constexpr (LOSS) {
+    // the forward's alpha is 1 - T_final exactly
+    const float om = 1.f - (1.f - tf);
+    const float cf[3] = {cf0, cf1, cf2};
+    float g[3], acc = 0.f;
+    ga = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float bgc = L.bg ? (okpix ? L.bg[3 * pix + c] : 0.f) : L.bg_value;
+      const float r = okpix ? fmaf(om, bgc, cf[c]) - L.target[3 * pix + c] : 0.f;
+      acc = fmaf(r, r, acc);
+      g[c] = 2.f * r * L.inv_count;
+      ga = fmaf(-g[c], bgc, ga);
+    }
+    gc0 = g[0]; gc1 = g[1]; gc2 = g[2];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (G.lane == 0 && L.loss) unsafeAtomicAdd(L.loss, acc * L.inv_count);
+  } else {
+    gc0 = okpix ? gCin[3 * pix] : 0.f;
+    gc1 = okpix ? gCin[3 * pix + 1] : 0.f;
+    gc2 = okpix ? gCin[3 * pix + 2] : 0.f;
+    ga = (okpix && gAin) ? gAin[pix] : 0.f;
+  }
   float T = 1.f, ca0 = 0.f, ca1 = 0.f, ca2 = 0.f;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
@@ -343,7 +395,7 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
       const Sample<DT> S = eval_sample<DT, true>(A, p, fpx, fpy, kk < lst);
       if (!__any(S.pushed)) continue;
       float col[3], dcx[3] = {0.f, 0.f, 0.f}, dcy[3] = {0.f, 0.f, 0.f};
-      if constexpr (BLEND) {
+      if (BLEND && A.cmap != nullptr) {
         sample_color_grad<DT>(A, p, sid[buf][j], S.tx, S.ty, col, dcx, dcy);
       } else {
         col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
@@ -376,7 +428,8 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
       // colour logits (tile_backward.cu:255-261); wgt = 0 for lanes not pushed.
       // With an exact colour-map gradient the blend weight (1 - c_blend)
       // applies; the reference leaves it out.
-      const float cw = (BLEND && A.exact_blend) ? wgt * (1.f - A.c_blend) : wgt;
+      const bool exact = BLEND && A.exact_blend && A.cmap != nullptr;
+      const float cw = exact ? wgt * (1.f - A.c_blend) : wgt;
       acc[5] = gc0 * cw * (p.cr * (1.f - p.cr));
       acc[6] = gc1 * cw * (p.cg * (1.f - p.cg));
       acc[7] = gc2 * cw * (p.cb * (1.f - p.cb));
@@ -388,7 +441,7 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
       float dLdu = dLdm * (S.gxt * A.hsx);
       float dLdv = dLdm * (S.gyt * A.hsy);
       if constexpr (BLEND) {
-        if (A.exact_blend) {
+        if (exact) {
           // colour-map term: dC/dcol_k = T_k a_k, dcol/d(tx,ty) = c_blend dc_o/d(tx,ty)
           const float kw = wgt * A.c_blend;
           dLdu = fmaf(kw * A.hsx, fmaf(gc0, dcx[0], fmaf(gc1, dcx[1], gc2 * dcx[2])), dLdu);
@@ -422,6 +475,18 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
     }
     __syncthreads();
   }
+}
+
+template <int DT, bool BLEND>
+__global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const float* __restrict__ gCin,
+                                                         const float* __restrict__ gAin,
+                                                         const float* __restrict__ Cfin,
+                                                         const float* __restrict__ Tfin,
+                                                         const int* __restrict__ lastin,
+                                                         float* __restrict__ grad8) {
+  const LossArgs none{};
+  raster_bwd_block<DT, BLEND, false>(A, xcd_remap(blockIdx.x, A.tiles * A.subs), gCin, gAin,
+                                     none, Cfin, Tfin, lastin, grad8);
 }
 
 }  // namespace dbmp

@@ -13,7 +13,7 @@ namespace dbmp {
 // result = C + (1 - A) * bg;  r = result - target;  loss += sum(r^2) * inv_count
 // dL/dC = 2 r * inv_count;  dL/dA = -sum_c dL/dC_c * bg_c.
 // bg is (H,W,3) or, when null, the constant bg_value.
-__global__ __launch_bounds__(256) void k_loss_mse(const float* __restrict__ color,
+static __global__ __launch_bounds__(256) void k_loss_mse(const float* __restrict__ color,
                                                   const float* __restrict__ alpha,
                                                   const float* __restrict__ bg, float bg_value,
                                                   const float* __restrict__ target, long long npix,
@@ -68,7 +68,7 @@ struct AdamArgs {
 //   m <- b1 m + (1-b1) g;  v <- b2 v + (1-b2) g^2
 //   p <- p - lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps)
 // blockIdx.y selects the parameter group.
-__global__ __launch_bounds__(256) void k_adam(AdamArgs A) {
+static __global__ __launch_bounds__(256) void k_adam(AdamArgs A) {
   const AdamGroup& G = A.g[blockIdx.y];
   const float ss = A.step_scale[blockIdx.y];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G.n;

@@ -31,14 +31,17 @@ class FitJob:
     is the storage of parameters and bitmaps seen by the rasterizer
     (torch.float32 / float16 / bfloat16); arithmetic, accumulation and the
     Adam master weights stay fp32.  ``bg`` is a (H,W,3) tensor or None for white.
-    ``max_steps`` sizes the on-device loss history.
+    ``max_steps`` sizes the on-device loss history.  ``exact_color_map_grad``
+    selects the exact gradient of the colour-map blend (``c_blend > 0``)
+    instead of the reference's.
     """
 
     def __init__(self, means2D, radii, rotations, opacities, colors, templates, bmp_sel, target,
                  *, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, store_dtype=torch.float32,
                  eps_T: float = 1e-6, tile_size: int = 16, alpha_upper_bound: float = 1.0,
                  clamp: bool = True, bg=None, max_steps: int = 1000, handle: Handle | None = None,
-                 colors_orig=None, c_blend: float = 0.0, colors_orig_per_template: bool = True):
+                 colors_orig=None, c_blend: float = 0.0, colors_orig_per_template: bool = True,
+                 exact_color_map_grad: bool = False):
         dev = means2D.device
         _native.require_gpu(means2D, "means2D")
         H, W = int(target.shape[0]), int(target.shape[1])
@@ -97,7 +100,7 @@ class FitJob:
             elif k == "radii":
                 g.lo[0], g.hi[0], g.lo[1], g.hi[1] = 2.0, rmax, 2.0, rmax
         self._bm = Handle.bitmaps(self.templates, self.colors_orig, self.co_mode, self.c_blend,
-                                  self.tmpl_version)
+                                  self.tmpl_version, exact_color_map_grad)
 
     def _prims(self, reuse: bool):
         s = self.store

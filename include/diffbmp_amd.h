@@ -190,6 +190,76 @@ int dbmp_timing_stats(dbmp_rasterizer* h, double* fwd_ms_total, int64_t* fwd_cou
                       double* bwd_ms_total, int64_t* bwd_count);
 int dbmp_timing_reset(dbmp_rasterizer* h);
 
+/* ------------------------------------------------------------------------
+ * Batched steps over many independent fit jobs (serving).
+ *
+ * The reference optimises one image per process (simple_tile_renderer.py:
+ * 1043-1140: binning, TileRasterizerFunction forward/backward, F.mse_loss,
+ * Adam, clamp); these entry points run that loop for a set of jobs at once,
+ * each job with its own canvas, primitive count, template set and target.
+ * One step launches each stage once for all jobs.  A job's parameters are
+ * rows of 8 values (x, y, r, theta, opacity logit, 3 colour logits) — the
+ * reference's means2D, radii, rotations, opacities and colors of one
+ * primitive side by side.
+ * ------------------------------------------------------------------------ */
+typedef struct dbmp_batch dbmp_batch;
+
+typedef struct {
+  int32_t height, width;        /* canvas */
+  int32_t num_primitives;
+  int32_t num_templates, tmpl_height, tmpl_width;
+  const void* quads;            /* prepared by dbmp_batch_prepare_bitmaps */
+  const float* ext;             /* (T,4) template extents, likewise */
+  const void* colors_orig;      /* (T,Ht,Wt,3) colour map per template, or NULL */
+  float c_blend;
+  int32_t blend_grad_exact;     /* as in dbmp_bitmaps */
+  const int32_t* bmp_sel;       /* (N) */
+  float* params;                /* (N,8) fp32 master */
+  float* exp_avg;               /* (N,8) */
+  float* exp_avg_sq;            /* (N,8) */
+  void* store;                  /* (N,8) in the batch's storage type (== params for fp32) */
+  const float* target;          /* (H,W,3) */
+  const float* background;      /* (H,W,3), or NULL for background_value */
+  float background_value;
+  float* out_color;             /* (H,W,3) scratch of the step */
+  float* out_alpha;             /* (H,W) */
+  float* out_T;                 /* (H,W) */
+  int32_t* out_last;            /* (H,W) */
+  float* loss_history;          /* (loss_capacity) MSE of step t at [t], zeroed by the caller */
+  int32_t loss_capacity;
+  float lr[5];                  /* means2D, radii, rotations, opacities, colors */
+  float alpha_upper_bound;
+  float eps_T;
+  int32_t clamp;                /* x in [0,W-1], y in [0,H-1], r in [2, min(H,W)/4] */
+  int32_t steps_done;           /* Adam steps already taken (the next step is steps_done+1) */
+} dbmp_job;
+
+/* storage_dtype: DBMP_F32 / F16 / BF16 for parameters, templates and colour
+ * maps of every job; arithmetic is fp32. */
+int dbmp_batch_create(dbmp_batch** out, int32_t storage_dtype, float beta1, float beta2, float eps);
+int dbmp_batch_destroy(dbmp_batch* b);
+
+/* Template extents (T,4) and texel quads (T*Ht*Wt entries of 16 bytes for
+ * fp32 storage, 8 bytes otherwise) of one template set, for dbmp_job. */
+int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext_out,
+                               void* quads_out, void* stream);
+
+/* Replace the job set (copies the descriptors; jobs keep their own device
+ * buffers).  pair_capacity bounds the (tile, primitive) pairs of all jobs
+ * together; a step whose pairs exceed it is still exact (every pixel walks all
+ * of its job's primitives) and reports the count through
+ * dbmp_batch_pair_count.  Waits for the batch's previous table upload. */
+int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
+                        int64_t pair_capacity, void* stream);
+
+/* One optimisation step of every job: binning, forward, background
+ * composite + MSE, backward, Adam, clamp.  use_graph: capture the step once
+ * per job set into a hipGraph and replay it. */
+int dbmp_batch_step(dbmp_batch* b, int32_t use_graph, void* stream);
+
+/* Pair count of the most recent step that has completed on the device. */
+int dbmp_batch_pair_count(dbmp_batch* b, int64_t* pairs);
+
 #ifdef __cplusplus
 }
 #endif

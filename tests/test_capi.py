@@ -45,6 +45,11 @@ def test_version_and_argument_errors_without_device():
     assert L.dbmp_init(None, 64, 64, 16, 0.0, 1.0, 151, 0) == 1
     assert L.dbmp_forward(None, None, None, None, 0.0, None, None, None, None, None) == 1
     assert L.dbmp_destroy(None) == 0
+    b = ctypes.c_void_p()
+    assert L.dbmp_batch_create(ctypes.byref(b), 7, 0.9, 0.999, 1e-8) == 1
+    assert b"storage dtype" in L.dbmp_last_error()
+    assert L.dbmp_batch_step(None, 0, None) == 1
+    assert L.dbmp_batch_destroy(None) == 0
 
 
 def test_struct_layouts_match_header():
@@ -52,9 +57,36 @@ def test_struct_layouts_match_header():
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     for cname, py in (("dbmp_prims", _native.Prims), ("dbmp_bitmaps", _native.Bitmaps),
                       ("dbmp_tile_lists", _native.TileLists), ("dbmp_adam_group", _native.AdamGroup),
-                      ("dbmp_adam", _native.Adam)):
+                      ("dbmp_adam", _native.Adam), ("dbmp_job", _native.Job)):
         body = re.search(r"typedef struct \{([^{}]*)\}\s*" + cname + ";", text).group(1)
         decls = [d for d in body.split(";") if d.strip()]
         n = sum(len(d.split(",")) for d in decls)
         # arrays (lo[2], groups[5]) are one ctypes field each
         assert n == len(py._fields_), (cname, n, len(py._fields_))
+
+
+def test_struct_offsets_match_c_compiler(tmp_path):
+    """Every ctypes mirror has the C compiler's size and field offsets."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"dbmp_prims": _native.Prims, "dbmp_bitmaps": _native.Bitmaps,
+               "dbmp_tile_lists": _native.TileLists, "dbmp_adam_group": _native.AdamGroup,
+               "dbmp_adam": _native.Adam, "dbmp_job": _native.Job}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, val = line.split()
+        py = structs[cname]
+        want = ctypes.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert int(val) == want, (cname, field, int(val), want)
