@@ -49,6 +49,16 @@ def parse():
     ap.add_argument("--psnr-steps", type=int, default=0,
                     help="also fit this many Adam steps at fp32 and at --precision from the same "
                          "start and report both final PSNRs")
+    ap.add_argument("--workload", default="scale", choices=["scale", "serving"],
+                    help="scale: the headline fit step; serving: a Poisson stream of fit jobs "
+                         "through the batched multi-job engine (its own JSON line)")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="arrival rate in jobs/s (0: every job arrives at t = 0, i.e. capacity)")
+    ap.add_argument("--serve-steps", type=int, default=200)
+    ap.add_argument("--serve-max-jobs", type=int, default=32)
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0)
+    ap.add_argument("--serve-no-graph", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "reference kernel for scaling HBM byte counters)")
@@ -212,6 +222,66 @@ def list_stats(handle, sc, dev, sdt, eps_T):
     return int(offs[-1].item()), float(lens.mean().item()), float(walked)
 
 
+def run_serving(args, dev):
+    """Poisson stream of fit jobs (canvas side uniform over {256, 512, 1024,
+    2048}, primitives log-uniform in [512, 32768], 1-16 of the stress
+    workload's 256^2 RGBA bitmaps, fp16 storage / fp32 arithmetic, 200 Adam
+    steps each to a smooth random field) through the batched engine."""
+    import torch
+    from diffbmp_amd import serve as SV
+
+    SV.template_library(256)          # host-side bitmap generation, outside the timed run
+    rng = np.random.default_rng(2024)
+    specs = [SV.random_spec(rng, steps=args.serve_steps) for _ in range(args.serve_jobs)]
+    if args.serve_rate > 0:
+        t = SV.poisson_arrivals(args.serve_rate, len(specs), rng)
+        for s, a in zip(specs, t):
+            s.arrival = float(a)
+    for i, s in enumerate(specs):
+        s.name = f"job{i}"
+    # warm-up: one small job end to end (kernel loading, graph capture path)
+    warm = SV.JobSpec(256, 256, 512, (0,), steps=5, seed=1, name="warm")
+    eng = SV.BatchEngine(store_dtype=torch.float16, use_graph=not args.serve_no_graph, device=dev)
+    SV.Server(eng, SV.HbmBudget(1 << 34)).run([warm])
+    eng = SV.BatchEngine(store_dtype=torch.float16, use_graph=not args.serve_no_graph, device=dev)
+    budget = SV.HbmBudget(int(args.serve_budget_gib * (1 << 30)))
+    srv = SV.Server(eng, budget, max_jobs=args.serve_max_jobs)
+    stats = srv.run(specs)
+    summ = stats.summary()
+    lat = np.array([r.latency for r in stats.results])
+    q = np.array([r.queue for r in stats.results])
+    prims = sum(s.num_primitives for s in specs)
+    px = sum(s.height * s.width for s in specs)
+    out = {
+        "metric": "serving jobs/s (200 Adam steps per job)",
+        "value": round(summ["jobs_per_s"], 3),
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f16-store/f32-acc",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson arrivals of fit jobs, canvas side U{256,512,1024,2048}, "
+                               "primitives log-U[512,32768], 1-16 RGBA 256^2 bitmaps, c_blend 0.5, "
+                               "fp16 storage, Adam lr 1e-2 on MSE to a smooth random field",
+                   "jobs": len(specs), "steps_per_job": args.serve_steps,
+                   "arrival_rate": args.serve_rate or "all at t=0",
+                   "max_jobs": args.serve_max_jobs, "hbm_budget_gib": args.serve_budget_gib,
+                   "graph": not args.serve_no_graph},
+        "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+        "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+        "queue_p50_s": round(float(np.percentile(q, 50)), 4),
+        "wall_s": round(summ["wall_s"], 3),
+        "batched_steps": summ["batched_steps"],
+        "ms_per_batched_step": round(summ["wall_s"] / max(summ["batched_steps"], 1) * 1e3, 4),
+        "max_active": summ["max_active"],
+        "total_primitives": prims,
+        "total_megapixels": round(px / 1e6, 2),
+        "mean_final_over_first_loss": round(float(np.mean([r.final_loss / r.first_loss
+                                                           for r in stats.results])), 4),
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -223,6 +293,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.workload == "serving":
+        run_serving(args, dev)
+        return
     if args.copy_probe:
         nb = args.copy_probe << 20
         src = torch.ones(nb // 4, device=dev)

@@ -50,7 +50,8 @@ class Bitmaps(ctypes.Structure):
                 ("num_templates", ctypes.c_int32), ("height", ctypes.c_int32),
                 ("width", ctypes.c_int32), ("dtype", ctypes.c_int32),
                 ("colors_orig_mode", ctypes.c_int32), ("c_blend", ctypes.c_float),
-                ("version", ctypes.c_int64), ("blend_grad_exact", ctypes.c_int32)]
+                ("version", ctypes.c_int64), ("blend_grad_exact", ctypes.c_int32),
+                ("colors_version", ctypes.c_int64)]
 
 
 class TileLists(ctypes.Structure):
@@ -64,7 +65,7 @@ class Job(ctypes.Structure):
                 ("num_primitives", ctypes.c_int32), ("num_templates", ctypes.c_int32),
                 ("tmpl_height", ctypes.c_int32), ("tmpl_width", ctypes.c_int32),
                 ("quads", ctypes.c_void_p), ("ext", ctypes.c_void_p),
-                ("colors_orig", ctypes.c_void_p), ("c_blend", ctypes.c_float),
+                ("color_quads", ctypes.c_void_p), ("c_blend", ctypes.c_float),
                 ("blend_grad_exact", ctypes.c_int32), ("bmp_sel", ctypes.c_void_p),
                 ("params", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
                 ("exp_avg_sq", ctypes.c_void_p), ("store", ctypes.c_void_p),
@@ -132,7 +133,7 @@ EXPORTS = {
     "dbmp_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "dbmp_batch_prepare_bitmaps": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Bitmaps),
                                                   ctypes.c_void_p, ctypes.c_void_p,
-                                                  ctypes.c_void_p]),
+                                                  ctypes.c_void_p, ctypes.c_void_p]),
     "dbmp_batch_set_jobs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Job), ctypes.c_int32,
                                            ctypes.c_int64, ctypes.c_void_p]),
     "dbmp_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
@@ -239,7 +240,8 @@ class Handle:
 
     @staticmethod
     def bitmaps(templates, colors_orig=None, colors_orig_mode=0, c_blend=0.0,
-                version: int = -1, blend_grad_exact: bool = False) -> Bitmaps:
+                version: int = -1, blend_grad_exact: bool = False,
+                colors_version: int = -1) -> Bitmaps:
         if templates.dim() != 3:
             raise ValueError("templates must be (T,Ht,Wt)")
         T, Ht, Wt = templates.shape
@@ -252,7 +254,7 @@ class Handle:
             co = colors_orig.data_ptr()
         return Bitmaps(templates.data_ptr(), co, T, Ht, Wt, _DT[templates.dtype],
                        int(colors_orig_mode if co else 0), float(c_blend), int(version),
-                       int(bool(blend_grad_exact)))
+                       int(bool(blend_grad_exact)), int(colors_version))
 
     def template_extents(self, templates: torch.Tensor) -> torch.Tensor:
         require_gpu(templates, "templates")

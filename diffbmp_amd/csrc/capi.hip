@@ -71,6 +71,12 @@ struct dbmp_rasterizer {
   const void* q_src = nullptr;
   long long q_version = -2;
   int q_T = 0, q_Ht = 0, q_Wt = 0, q_dt = -1;
+  // colour-map quads cache (per-template maps)
+  void* cquads = nullptr;
+  size_t cquads_bytes = 0;
+  const void* cq_src = nullptr;
+  long long cq_version = -2;
+  int cq_T = 0, cq_Ht = 0, cq_Wt = 0, cq_dt = -1;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_fwd, ev_bwd;
@@ -180,6 +186,35 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   return 0;
 }
 
+// Colour-map quads of per-template colour maps (mode 2), rebuilt when the
+// maps' pointer, version (-1: every call), shape or type changes.
+int ensure_cquads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
+  if (b->colors_orig_mode != 2 || !(b->c_blend > 0.f)) return 0;
+  const long long tot = (long long)b->num_templates * b->height * b->width;
+  const size_t bytes = (size_t)tot * cquad_bytes(b->dtype);
+  if (b->colors_version >= 0 && h->cq_src == b->colors_orig && h->cq_version == b->colors_version &&
+      h->cq_T == b->num_templates && h->cq_Ht == b->height && h->cq_Wt == b->width &&
+      h->cq_dt == b->dtype)
+    return 0;
+  if (bytes > h->cquads_bytes) {
+    if (h->cquads) (void)hipFree(h->cquads);
+    h->cquads = nullptr;
+    h->cquads_bytes = 0;
+    hipError_t e = hipMalloc(&h->cquads, bytes);
+    if (e != hipSuccess) return fail(DBMP_E_ALLOC, "hipMalloc colour quads: %s", hipGetErrorString(e));
+    h->cquads_bytes = bytes;
+  }
+  int rc = launch_build_cquads(b, h->cquads, st);
+  if (rc) return rc;
+  h->cq_src = b->colors_orig;
+  h->cq_version = b->colors_version;
+  h->cq_T = b->num_templates;
+  h->cq_Ht = b->height;
+  h->cq_Wt = b->width;
+  h->cq_dt = b->dtype;
+  return 0;
+}
+
 RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps* b,
                      const dbmp_tile_lists* L, float eps) {
   RasterArgs A;
@@ -190,8 +225,9 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.pcap = L->capacity;
   A.quads = h->quads;
   const bool blend = b->colors_orig_mode != 0 && b->c_blend > 0.f;
-  A.cmap = blend ? b->colors_orig : nullptr;
-  A.cmap_per_prim = b->colors_orig_mode == 1;
+  // per-template maps are sampled through their colour-map quads
+  A.cmap = !blend ? nullptr : (b->colors_orig_mode == 2 ? h->cquads : b->colors_orig);
+  A.cmap_mode = b->colors_orig_mode == 2 ? 2 : 1;
   A.c_blend = b->c_blend;
   A.exact_blend = b->blend_grad_exact != 0;
   A.N = p->num_primitives;
@@ -291,7 +327,7 @@ int dbmp_destroy(dbmp_rasterizer* h) {
   timing_drain(h->ev_fwd, ms, n);
   timing_drain(h->ev_bwd, ms, n);
   void* ptrs[] = {h->recs, h->counts, h->offs, h->ranges, h->grad8, h->keysA,
-                  h->keysB, h->valsB, h->hist, h->totals, h->quads};
+                  h->keysB, h->valsB, h->hist, h->totals, h->quads, h->cquads};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -402,6 +438,7 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
   timing_begin(h, h->ev_fwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
+  if ((rc = ensure_cquads(h, bm, st))) return rc;
   // raster constants only (no extents => no tile ranges)
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
   const RasterArgs A = make_args(h, prims, bm, lists, eps_T);
@@ -432,6 +469,7 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
   timing_begin(h, h->ev_bwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
+  if ((rc = ensure_cquads(h, bm, st))) return rc;
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
   if (N > 0) {
     hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);

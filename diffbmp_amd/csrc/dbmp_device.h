@@ -68,6 +68,39 @@ template <> struct QuadT<kBF16> {
   }
 };
 
+// Colour-map quads: entry (m, y, x) holds the RGB of texels (x, y), (x1, y),
+// (x, y1), (x1, y1) with x1 = min(x+1, W-1), y1 = min(y+1, H-1) (clamp to
+// edge), as 12 values padded to 16: one aligned 64-byte (fp32) or 32-byte
+// (fp16 / bf16) record per bilinear colour sample.
+template <int DT> struct CQuad;
+template <> struct CQuad<kF32> {
+  using type = float4;
+  __device__ static __forceinline__ void get(const float4* q, long i, float c00[3], float c10[3],
+                                             float c01[3], float c11[3]) {
+    const float4 a = q[4 * i], b = q[4 * i + 1], c = q[4 * i + 2];
+    c00[0] = a.x; c00[1] = a.y; c00[2] = a.z;
+    c10[0] = a.w; c10[1] = b.x; c10[2] = b.y;
+    c01[0] = b.z; c01[1] = b.w; c01[2] = c.x;
+    c11[0] = c.y; c11[1] = c.z; c11[2] = c.w;
+  }
+};
+template <int DT>
+struct CQuadHalf {
+  using type = uint4;
+  __device__ static __forceinline__ float lo(uint32_t v) { return DT == kF16 ? f16_lo(v) : bf16_lo(v); }
+  __device__ static __forceinline__ float hi(uint32_t v) { return DT == kF16 ? f16_hi(v) : bf16_hi(v); }
+  __device__ static __forceinline__ void get(const uint4* q, long i, float c00[3], float c10[3],
+                                             float c01[3], float c11[3]) {
+    const uint4 a = q[2 * i], b = q[2 * i + 1];
+    c00[0] = lo(a.x); c00[1] = hi(a.x); c00[2] = lo(a.y);
+    c10[0] = hi(a.y); c10[1] = lo(a.z); c10[2] = hi(a.z);
+    c01[0] = lo(a.w); c01[1] = hi(a.w); c01[2] = lo(b.x);
+    c11[0] = hi(b.x); c11[1] = lo(b.y); c11[2] = hi(b.y);
+  }
+};
+template <> struct CQuad<kF16> : CQuadHalf<kF16> {};
+template <> struct CQuad<kBF16> : CQuadHalf<kBF16> {};
+
 // sigmoidf_safe (tile_common.h:125-133); evaluated once per primitive
 __device__ __forceinline__ float sigmoid_safe(float x) {
   if (x >= 0.f) {
@@ -126,53 +159,6 @@ __device__ __forceinline__ float bilerp_grad(float v00, float v01, float v10, fl
   gxt = fmaf(wy, dxb - dxa, dxa);
   gyt = bot - top;
   return fmaf(wy, bot - top, top);
-}
-
-// Clamp-to-edge bilinear RGB colour map (tile_forward.cu:17-89); the
-// coordinates are already inside [0,W-1] x [0,H-1] for a pushed sample.
-template <typename CT>
-__device__ __forceinline__ void color_map_sample(const CT* cm, int Ht, int Wt, float cx, float cy,
-                                                 float out[3]) {
-  cx = fmaxf(0.f, fminf((float)(Wt - 1), cx));
-  cy = fmaxf(0.f, fminf((float)(Ht - 1), cy));
-  const int x0 = (int)cx, y0 = (int)cy;
-  const int x1 = min(x0 + 1, Wt - 1), y1 = min(y0 + 1, Ht - 1);
-  const float fx = cx - (float)x0, fy = cy - (float)y0;
-#pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    const float c00 = ldf(cm, 3L * (y0 * Wt + x0) + ch);
-    const float c01 = ldf(cm, 3L * (y1 * Wt + x0) + ch);
-    const float c10 = ldf(cm, 3L * (y0 * Wt + x1) + ch);
-    const float c11 = ldf(cm, 3L * (y1 * Wt + x1) + ch);
-    const float c0 = fmaf(fx, c10 - c00, c00);
-    const float c1 = fmaf(fx, c11 - c01, c01);
-    out[ch] = fmaf(fy, c1 - c0, c0);
-  }
-}
-
-// The same colour-map sample and its derivative w.r.t. (cx, cy); zero in a
-// direction where the coordinate sits on the last texel (clamp-to-edge).
-template <typename CT>
-__device__ __forceinline__ void color_map_sample_grad(const CT* cm, int Ht, int Wt, float cx,
-                                                      float cy, float out[3], float dcx[3],
-                                                      float dcy[3]) {
-  cx = fmaxf(0.f, fminf((float)(Wt - 1), cx));
-  cy = fmaxf(0.f, fminf((float)(Ht - 1), cy));
-  const int x0 = (int)cx, y0 = (int)cy;
-  const int x1 = min(x0 + 1, Wt - 1), y1 = min(y0 + 1, Ht - 1);
-  const float fx = cx - (float)x0, fy = cy - (float)y0;
-#pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    const float c00 = ldf(cm, 3L * (y0 * Wt + x0) + ch);
-    const float c01 = ldf(cm, 3L * (y1 * Wt + x0) + ch);
-    const float c10 = ldf(cm, 3L * (y0 * Wt + x1) + ch);
-    const float c11 = ldf(cm, 3L * (y1 * Wt + x1) + ch);
-    const float c0 = fmaf(fx, c10 - c00, c00);
-    const float c1 = fmaf(fx, c11 - c01, c01);
-    out[ch] = fmaf(fy, c1 - c0, c0);
-    dcx[ch] = fmaf(fy, (c11 - c01) - (c10 - c00), c10 - c00);
-    dcy[ch] = c1 - c0;
-  }
 }
 
 // Wave-level helpers (wave64) ---------------------------------------------

@@ -36,8 +36,9 @@ struct RasterArgs {
   const long long* pcount;  // binning pair count (device) or null
   long long pcap;           // capacity the lists were built with
   const void* quads;
-  const void* cmap;      // colour maps (M,Ht,Wt,3) or null
-  int cmap_per_prim;     // 1: M = N (indexed by primitive), 0: M = T (by template)
+  const void* cmap;      // colour maps or null; layout by cmap_mode
+  int cmap_mode;         // 0: (T,Ht,Wt,3) by template, 1: (N,Ht,Wt,3) by primitive,
+                         // 2: colour-map quads (T,Ht,Wt) of the per-template maps
   float c_blend;
   int exact_blend;       // 1: gradient of the blended colour w.r.t. logits and geometry;
                          // 0: the reference's (no (1 - c_blend), no colour-map term)
@@ -61,37 +62,70 @@ template <int DT>
 using StoreT = typename std::conditional<DT == kF32, float,
                typename std::conditional<DT == kF16, __half, __hip_bfloat16>::type>::type;
 
-// Blended colour and its derivative w.r.t. the texture coordinates (tx, ty).
+// The four clamp-to-edge neighbours of a colour-map sample
+// (sample_color_map, tile_forward.cu:17-89) and the bilinear weights.
+struct CCorners {
+  float fx, fy;
+  float c00[3], c10[3], c01[3], c11[3];   // (x0,y0), (x1,y0), (x0,y1), (x1,y1)
+};
+
 template <int DT>
-__device__ __forceinline__ void sample_color_grad(const RasterArgs& A, const PrimRec& p, int n,
-                                                  float tx, float ty, float col[3], float dcx[3],
-                                                  float dcy[3]) {
-  const long m = A.cmap_per_prim ? (long)n : (long)p.tmpl;
-  const StoreT<DT>* cm = (const StoreT<DT>*)A.cmap + m * (long)A.Ht * A.Wt * 3;
-  float co[3];
-  color_map_sample_grad(cm, A.Ht, A.Wt, tx, ty, co, dcx, dcy);
-  const float k = A.c_blend, k1 = 1.f - A.c_blend;
-  col[0] = k1 * p.cr + k * co[0];
-  col[1] = k1 * p.cg + k * co[1];
-  col[2] = k1 * p.cb + k * co[2];
+__device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const PrimRec& p, int n,
+                                                 float cx, float cy) {
+  CCorners k;
+  cx = fmaxf(0.f, fminf(A.Wtm1, cx));
+  cy = fmaxf(0.f, fminf(A.Htm1, cy));
+  const int x0 = (int)cx, y0 = (int)cy;
+  k.fx = cx - (float)x0;
+  k.fy = cy - (float)y0;
+  if (A.cmap_mode == 2) {
+    using Q = CQuad<DT>;
+    const long i = ((long)p.tmpl * A.Ht + y0) * A.Wt + x0;
+    Q::get((const typename Q::type*)A.cmap, i, k.c00, k.c10, k.c01, k.c11);
+  } else {
+    const long m = A.cmap_mode == 1 ? (long)n : (long)p.tmpl;
+    const StoreT<DT>* cm = (const StoreT<DT>*)A.cmap + m * (long)A.Ht * A.Wt * 3;
+    const int x1 = min(x0 + 1, A.Wt - 1), y1 = min(y0 + 1, A.Ht - 1);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      k.c00[ch] = ldf(cm, 3L * (y0 * A.Wt + x0) + ch);
+      k.c01[ch] = ldf(cm, 3L * (y1 * A.Wt + x0) + ch);
+      k.c10[ch] = ldf(cm, 3L * (y0 * A.Wt + x1) + ch);
+      k.c11[ch] = ldf(cm, 3L * (y1 * A.Wt + x1) + ch);
+    }
+  }
+  return k;
+}
+
+// Blended colour (1 - k) sig(c) + k c_o and, with GRAD, dc_o/d(tx, ty).
+template <int DT, bool GRAD>
+__device__ __forceinline__ void blend_color(const RasterArgs& A, const PrimRec& p, int n, float tx,
+                                            float ty, float col[3], float dcx[3], float dcy[3]) {
+  const CCorners k = cmap_corners<DT>(A, p, n, tx, ty);
+  const float pc[3] = {p.cr, p.cg, p.cb};
+  const float kb = A.c_blend, k1 = 1.f - A.c_blend;
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    const float c0 = fmaf(k.fx, k.c10[ch] - k.c00[ch], k.c00[ch]);
+    const float c1 = fmaf(k.fx, k.c11[ch] - k.c01[ch], k.c01[ch]);
+    const float co = fmaf(k.fy, c1 - c0, c0);
+    col[ch] = k1 * pc[ch] + kb * co;
+    if constexpr (GRAD) {
+      dcx[ch] = fmaf(k.fy, (k.c11[ch] - k.c01[ch]) - (k.c10[ch] - k.c00[ch]), k.c10[ch] - k.c00[ch]);
+      dcy[ch] = c1 - c0;
+    }
+  }
 }
 
 template <int DT, bool BLEND>
 __device__ __forceinline__ void sample_color(const RasterArgs& A, const PrimRec& p, int n,
-                                             const Geo& g, float col[3]) {
-  col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
+                                             float tx, float ty, float col[3]) {
   // a batch mixing canvases with and without colour maps runs BLEND kernels;
   // a canvas without one keeps its plain colours
   if (BLEND && A.cmap != nullptr) {
-    using CT = typename std::conditional<DT == kF32, float,
-               typename std::conditional<DT == kF16, __half, __hip_bfloat16>::type>::type;
-    const long m = A.cmap_per_prim ? (long)n : (long)p.tmpl;
-    const CT* cm = (const CT*)A.cmap + m * (long)A.Ht * A.Wt * 3;
-    float co[3];
-    color_map_sample(cm, A.Ht, A.Wt, g.tx, g.ty, co);
-    const float k = A.c_blend, k1 = 1.f - A.c_blend;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) col[ch] = k1 * col[ch] + k * co[ch];
+    blend_color<DT, false>(A, p, n, tx, ty, col, nullptr, nullptr);
+  } else {
+    col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
   }
 }
 
@@ -257,11 +291,12 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         if (!(__builtin_amdgcn_readfirstlane(smask[buf][j]) & wbit)) continue;
         const PrimRec p = srec[buf][j];
         const Sample<DT> S = eval_sample<DT, false>(A, p, fpx, fpy, alive);
+        if constexpr (BLEND) {
+          // skip the colour-map gathers when nothing of this entry reaches the wave
+          if (!__any(S.pushed)) continue;
+        }
         float col[3];
-        Geo geo;
-        geo.tx = S.tx;
-        geo.ty = S.ty;
-        sample_color<DT, BLEND>(A, p, sid[buf][j], geo, col);
+        sample_color<DT, BLEND>(A, p, sid[buf][j], S.tx, S.ty, col);
         const float a = S.pushed ? p.abase * S.m : 0.f;   // a = 0: compositing is a no-op
         const float wgt = T * a;
         C0 = fmaf(wgt, col[0], C0);
@@ -395,8 +430,9 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const Sample<DT> S = eval_sample<DT, true>(A, p, fpx, fpy, kk < lst);
       if (!__any(S.pushed)) continue;
       float col[3], dcx[3] = {0.f, 0.f, 0.f}, dcy[3] = {0.f, 0.f, 0.f};
-      if (BLEND && A.cmap != nullptr) {
-        sample_color_grad<DT>(A, p, sid[buf][j], S.tx, S.ty, col, dcx, dcy);
+      const bool has_map = BLEND && A.cmap != nullptr;
+      if (has_map) {
+        blend_color<DT, true>(A, p, sid[buf][j], S.tx, S.ty, col, dcx, dcy);
       } else {
         col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
       }
@@ -428,7 +464,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       // colour logits (tile_backward.cu:255-261); wgt = 0 for lanes not pushed.
       // With an exact colour-map gradient the blend weight (1 - c_blend)
       // applies; the reference leaves it out.
-      const bool exact = BLEND && A.exact_blend && A.cmap != nullptr;
+      const bool exact = has_map && A.exact_blend;
       const float cw = exact ? wgt * (1.f - A.c_blend) : wgt;
       acc[5] = gc0 * cw * (p.cr * (1.f - p.cr));
       acc[6] = gc1 * cw * (p.cg * (1.f - p.cg));

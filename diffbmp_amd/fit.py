@@ -67,7 +67,8 @@ class FitJob:
             templates.to(dev), store_dtype)
         self.colors_orig, self.co_mode, self.c_blend = None, 0, float(c_blend)
         if colors_orig is not None and c_blend > 0:
-            self.colors_orig = colors_orig.to(dev, store_dtype).contiguous()
+            # owned copy: the colour-map quads built from it are cached for the job
+            self.colors_orig = colors_orig.to(dev, store_dtype).contiguous().clone()
             self.co_mode = 2 if colors_orig_per_template else 1
         self.target = target.to(dev, f32).contiguous()
         self.bg = None if bg is None else bg.to(dev, f32).contiguous()
@@ -100,7 +101,7 @@ class FitJob:
             elif k == "radii":
                 g.lo[0], g.hi[0], g.lo[1], g.hi[1] = 2.0, rmax, 2.0, rmax
         self._bm = Handle.bitmaps(self.templates, self.colors_orig, self.co_mode, self.c_blend,
-                                  self.tmpl_version, exact_color_map_grad)
+                                  self.tmpl_version, exact_color_map_grad, colors_version=0)
 
     def _prims(self, reuse: bool):
         s = self.store

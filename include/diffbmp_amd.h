@@ -69,6 +69,8 @@ typedef struct {
   int32_t blend_grad_exact; /* 0: the reference's colour gradient under c_blend > 0
                                (tile_backward.cu:253-258: no (1 - c_blend) factor, no
                                dependence of the colour map on geometry); 1: exact */
+  int64_t colors_version;   /* caller's content version of per-template `colors_orig`
+                               (mode 2); -1 = unknown */
 } dbmp_bitmaps;
 
 /* CSR tile lists: tile_offsets (tiles+1), tile_indices (P).  When the lists
@@ -210,7 +212,7 @@ typedef struct {
   int32_t num_templates, tmpl_height, tmpl_width;
   const void* quads;            /* prepared by dbmp_batch_prepare_bitmaps */
   const float* ext;             /* (T,4) template extents, likewise */
-  const void* colors_orig;      /* (T,Ht,Wt,3) colour map per template, or NULL */
+  const void* color_quads;      /* colour-map quads of the per-template maps, likewise, or NULL */
   float c_blend;
   int32_t blend_grad_exact;     /* as in dbmp_bitmaps */
   const int32_t* bmp_sel;       /* (N) */
@@ -239,10 +241,12 @@ typedef struct {
 int dbmp_batch_create(dbmp_batch** out, int32_t storage_dtype, float beta1, float beta2, float eps);
 int dbmp_batch_destroy(dbmp_batch* b);
 
-/* Template extents (T,4) and texel quads (T*Ht*Wt entries of 16 bytes for
- * fp32 storage, 8 bytes otherwise) of one template set, for dbmp_job. */
+/* Template extents (T,4), texel quads (T*Ht*Wt entries of 16 bytes for fp32
+ * storage, 8 bytes otherwise) and, when bm carries per-template colour maps
+ * (colors_orig_mode 2), colour-map quads (T*Ht*Wt entries of 64 / 32 bytes)
+ * of one template set, for dbmp_job. */
 int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext_out,
-                               void* quads_out, void* stream);
+                               void* quads_out, void* color_quads_out, void* stream);
 
 /* Replace the job set (copies the descriptors; jobs keep their own device
  * buffers).  pair_capacity bounds the (tile, primitive) pairs of all jobs
