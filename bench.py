@@ -314,6 +314,13 @@ def main():
     eps_T = args.eps_t if args.eps_t is not None else 1e-6
     P, list_mean, walked = list_stats(res["handle"], sc, dev, sdt, eps_T)
 
+    traffic = None
+    prof = os.path.join(HERE, "profiles", "pmc_traffic_scale_fp32.json")
+    if args.precision == "fp32" and os.path.exists(prof):
+        # HBM bytes per backward launch from the committed FETCH_SIZE / WRITE_SIZE
+        # passes of this workload, scaled by the copy-probe factors recorded there
+        with open(prof) as f:
+            traffic = json.load(f)["kernels"]["k_raster_bwd"]["traffic_bytes"]
     b_p = torch.tensor([], dtype=sdt).element_size()
     b_bin, b_fwd, b_bwd = algorithmic_bytes(H, W, N, T, Ht, Wt, P, res["handle"].tiles, b_p, b_p)
     achieved = b_bwd / res["bwd_s"]
@@ -345,7 +352,8 @@ def main():
         "list_len_mean": round(list_mean, 2),
         "entries_walked_per_pixel": round(walked, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
+                     "traffic_source": "profiles/pmc_traffic_scale_fp32.json" if traffic else None,
                      "kernel": "k_raster_bwd", "algorithmic_bytes": int(b_bwd),
                      "step_algorithmic_bytes": int(b_bin + b_fwd + b_bwd)},
     }
