@@ -359,7 +359,7 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.N = s.num_primitives;
     A.Ht = s.tmpl_height;
     A.Wt = s.tmpl_width;
-    A.qmax = s.num_templates * s.tmpl_height * s.tmpl_width - 1;
+    A.HWm1f = (float)(s.tmpl_height * s.tmpl_width - 1);
     A.Wtm1 = (float)(s.tmpl_width - 1);
     A.Htm1 = (float)(s.tmpl_height - 1);
     A.hsx = 0.5f * (float)(s.tmpl_width - 1);

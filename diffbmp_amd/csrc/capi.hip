@@ -233,7 +233,7 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.N = p->num_primitives;
   A.Ht = b->height;
   A.Wt = b->width;
-  A.qmax = b->num_templates * b->height * b->width - 1;
+  A.HWm1f = (float)(b->height * b->width - 1);
   A.Wtm1 = (float)(b->width - 1);
   A.Htm1 = (float)(b->height - 1);
   A.hsx = 0.5f * (float)(b->width - 1);

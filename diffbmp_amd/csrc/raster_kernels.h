@@ -43,7 +43,7 @@ struct RasterArgs {
   int exact_blend;       // 1: gradient of the blended colour w.r.t. logits and geometry;
                          // 0: the reference's (no (1 - c_blend), no colour-map term)
   int N, Ht, Wt;
-  int qmax;              // last valid texel-quad index (T*Ht*Wt - 1)
+  float HWm1f;           // Ht*Wt - 1: last texel of one template, as a float
   float Wtm1, Htm1, hsx, hsy;
   int H, W, ts, tilesX, tiles, subs;   // subs = (ts/16)^2 sub-tiles per tile
   float eps;
@@ -145,7 +145,9 @@ __device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A, int b) {
   const int tx = g.tile % A.tilesX, ty = g.tile / A.tilesX;
   g.bx = tx * A.ts + (sub % spr) * kSub;
   g.by = ty * A.ts + (sub / spr) * kSub;
-  g.w = threadIdx.x >> 6;
+  // wave-uniform by construction; readfirstlane lets the compiler keep the
+  // cull tests and loop counters in scalar registers
+  g.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   g.lane = threadIdx.x & 63;
   g.px = g.bx + (g.lane & 15);
   g.py = g.by + 4 * g.w + (g.lane >> 4);
@@ -228,19 +230,23 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   using Q = QuadT<DT>;
   Sample<DT> r;
   const float dx = fpx - p.mx, dy = fpy - p.my;
-  const bool inbox = fabsf(dx) <= p.hw && fabsf(dy) <= p.hh;
+  // predicates combined with & (no short-circuit branches)
+  const bool inbox = (fabsf(dx) <= p.hw) & (fabsf(dy) <= p.hh);
   r.ndx = dx * p.rinv;
   r.ndy = dy * p.rinv;
   r.u = p.c * r.ndx + p.s * r.ndy;
   r.v = -p.s * r.ndx + p.c * r.ndy;
   r.tx = (r.u + 1.f) * A.hsx;
   r.ty = (r.v + 1.f) * A.hsy;
-  const bool inb = r.tx >= 0.f && r.tx <= A.Wtm1 && r.ty >= 0.f && r.ty <= A.Htm1;
-  const bool ok = active && inbox && inb;
+  const bool inb = (r.tx >= 0.f) & (r.tx <= A.Wtm1) & (r.ty >= 0.f) & (r.ty <= A.Htm1);
+  const bool ok = active & inbox & inb;
   const float fx = floorf(r.tx), fy = floorf(r.ty);
   // clamped rather than selected, so the gather is unconditional; a lane
   // that is not `ok` reads some valid quad and its sample is discarded
-  const int idx = min(max(p.tbase + (int)fy * A.Wt + (int)fx, 0), A.qmax);
+  // texel index within the template in float (exact below 2^24), clamped so
+  // that NaN or out-of-range coordinates still address a texel of the template
+  const float fl = fminf(fmaxf(fmaf(fy, A.Wtm1 + 1.f, fx), 0.f), A.HWm1f);
+  const int idx = p.tbase + (int)fl;
   float v00, v01, v10, v11;
   Q::get((const typename Q::type*)A.quads, idx, v00, v01, v10, v11);
   const float wx = r.tx - fx, wy = r.ty - fy;
@@ -252,7 +258,7 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
     r.gxt = r.gyt = 0.f;
   }
   r.m = fmaxf(0.f, fminf(1.f, m));
-  r.pushed = ok && r.m > 1e-6f;
+  r.pushed = ok & (r.m > 1e-6f);
   return r;
 }
 
@@ -285,7 +291,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     const bool have_next = base + kBatch < end;
     Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
-    const int cnt = min(kBatch, end - base);
+    const int cnt = __builtin_amdgcn_readfirstlane(min(kBatch, end - base));
     if (__any(alive)) {
       for (int j = 0; j < cnt; ++j) {
         if (!(__builtin_amdgcn_readfirstlane(smask[buf][j]) & wbit)) continue;
@@ -421,7 +427,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     const bool have_next = base + kBatch < stop;
     Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
-    const int cnt = min(kBatch, stop - base);
+    const int cnt = __builtin_amdgcn_readfirstlane(min(kBatch, stop - base));
     for (int j = 0; j < cnt; ++j) {
       const int kk = base - start + j;
       if (kk >= wkmax) break;
