@@ -412,7 +412,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   float T = 1.f, ca0 = 0.f, ca1 = 0.f, ca2 = 0.f;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
-  const int wkmax = wave_max_i32(lst);
+  const int wkmax = __builtin_amdgcn_readfirstlane(wave_max_i32(lst));
   if (threadIdx.x == 0) skmax = 0;
   for (int i = threadIdx.x; i < 4 * kBatch * 8; i += kThreads) (&sred[0][0][0])[i] = 0.f;
   __syncthreads();
@@ -458,10 +458,10 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       // suffix transmittance B_k = prod_{m>k} (1 - a_m) = T_final / T_{k+1};
       // for an exactly opaque layer (fk = 0) it is the nonzero-factor product
       // after it, tnzf / T_k, if it is the only opaque layer, else 0
-      float B;
-      if (Tn > 0.f) B = fminf(tf * __builtin_amdgcn_rcpf(Tn), 1.f);
-      else if (fk == 0.f) B = nzf == 1 ? fminf(tnzf * __builtin_amdgcn_rcpf(Tk), 1.f) : 0.f;
-      else B = 1.f;
+      // (selects, one reciprocal per lane)
+      const bool tpos = Tn > 0.f;
+      const float rB = fminf((tpos ? tf : tnzf) * __builtin_amdgcn_rcpf(tpos ? Tn : Tk), 1.f);
+      const float B = tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f);
       float dLda = ga * (Tk * B);
       dLda = fmaf(gc0, fmaf(Tk, col[0], -S0 * inv1m), dLda);
       dLda = fmaf(gc1, fmaf(Tk, col[1], -S1 * inv1m), dLda);
