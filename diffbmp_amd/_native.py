@@ -121,6 +121,13 @@ EXPORTS = {
                                      ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p]),
+    "dbmp_backward_mse": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims),
+                                         ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
+                                         ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
     "dbmp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Adam), ctypes.c_void_p]),
     "dbmp_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dbmp_timing_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
@@ -356,6 +363,17 @@ class Handle:
                                     float(eps_T), _ptr(grad_color), _ptr(grad_alpha), _ptr(out_color),
                                     _ptr(out_T), _ptr(out_last), _ptr(g_means), _ptr(g_r), _ptr(g_th),
                                     _ptr(g_v), _ptr(g_c), _stream(grad_color.device)))
+
+    def backward_mse(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T, target, out_color, out_T,
+                     out_last, g_means, g_r, g_th, g_v, g_c, loss_accum=None, bg=None,
+                     bg_value: float = 1.0) -> None:
+        """Background composite + MSE + backward in one raster pass."""
+        L = lists.c()
+        check(self._L.dbmp_backward_mse(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(L),
+                                        float(eps_T), _ptr(target), _ptr(bg), float(bg_value),
+                                        _ptr(loss_accum), _ptr(out_color), _ptr(out_T), _ptr(out_last),
+                                        _ptr(g_means), _ptr(g_r), _ptr(g_th), _ptr(g_v), _ptr(g_c),
+                                        _stream(target.device)))
 
     def loss_mse(self, color, alpha, target, grad_color, grad_alpha, loss_accum=None, bg=None,
                  bg_value: float = 1.0) -> None:

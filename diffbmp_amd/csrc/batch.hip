@@ -141,7 +141,7 @@ int launch_step(dbmp_batch* b, hipStream_t st) {
                        prim_start(b), J, b->recs, b->counts, b->ranges);
     LAUNCH_CHECK("k_batch_prep");
   }
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(256), 0, st, b->counts, N, b->offs, b->dtot);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, b->counts, N, b->offs, b->dtot);
   LAUNCH_CHECK("k_scan_counts");
   int bits = 0;
   while ((1 << bits) < b->tiles_tot) ++bits;

@@ -6,7 +6,7 @@
 //   k_prep        one thread per primitive: rotated template-extent AABB in
 //                 exact fp32 (every torch op rounded separately), closed-interval
 //                 tile range, pair count, and the raster constants (PrimRec).
-//   k_scan_counts exclusive 64-bit scan of the pair counts (one workgroup).
+//   k_scan_counts exclusive 64-bit scan of the pair counts (one 1024-thread workgroup).
 //   k_emit        one wave per primitive writes its (tile id, primitive id)
 //                 pairs, tiles in row-major order, primitives in id order.
 //   k_radix_*     stable LSD radix sort on the tile id; stability keeps the
@@ -201,24 +201,52 @@ __device__ __forceinline__ long long block256_excl_scan(long long v, long long& 
   return inc - v + before;
 }
 
-// offs[i] = sum(counts[0..i)), offs[N] = *total = sum(counts).
-static __global__ __launch_bounds__(256) void k_scan_counts(const int* __restrict__ counts, int N,
-                                                     long long* __restrict__ offs,
-                                                     long long* __restrict__ total) {
-  __shared__ long long sh[4];
+// Exclusive scan over a block of NW waves (sh: NW slots).
+template <int NW>
+__device__ __forceinline__ long long block_excl_scan(long long v, long long& total, long long* sh) {
+  const int w = threadIdx.x >> 6, l = lane_id();
+  const long long inc = wave_incl_scan_i64(v);
+  if (l == 63) sh[w] = inc;
+  __syncthreads();
+  long long before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    const long long s = sh[i];
+    before += (i < w) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return inc - v + before;
+}
+
+// offs[i] = sum(counts[0..i)), offs[N] = *total = sum(counts).  One
+// 1024-thread workgroup, 8 counts per thread per round (8192 per round).
+constexpr int kScanThreads = 1024;
+static __global__ __launch_bounds__(kScanThreads) void k_scan_counts(const int* __restrict__ counts,
+                                                                     int N,
+                                                                     long long* __restrict__ offs,
+                                                                     long long* __restrict__ total) {
+  __shared__ long long sh[kScanThreads / 64];
   constexpr int kItems = 8;
   long long carry = 0;
-  for (int base = 0; base < N; base += 256 * kItems) {
+  for (int base = 0; base < N; base += kScanThreads * kItems) {
     const int i0 = base + threadIdx.x * kItems;
-    long long loc[kItems];
+    int loc[kItems];
     long long s = 0;
+    if (i0 + kItems <= N) {
+      const int4 a = *reinterpret_cast<const int4*>(counts + i0);
+      const int4 b = *reinterpret_cast<const int4*>(counts + i0 + 4);
+      loc[0] = a.x; loc[1] = a.y; loc[2] = a.z; loc[3] = a.w;
+      loc[4] = b.x; loc[5] = b.y; loc[6] = b.z; loc[7] = b.w;
+    } else {
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      loc[j] = (i0 + j < N) ? (long long)counts[i0 + j] : 0;
-      s += loc[j];
+      for (int j = 0; j < kItems; ++j) loc[j] = (i0 + j < N) ? counts[i0 + j] : 0;
     }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) s += loc[j];
     long long tot;
-    long long ex = block256_excl_scan(s, tot, sh) + carry;
+    long long ex = block_excl_scan<kScanThreads / 64>(s, tot, sh) + carry;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
       if (i0 + j < N) offs[i0 + j] = ex;

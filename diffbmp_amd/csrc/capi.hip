@@ -371,7 +371,7 @@ int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int 
   if ((rc = ensure_pairs(h, std::max<long long>(capacity, 1)))) return rc;
   long long* dtot = (long long*)d_pair_count;
   if ((rc = launch_prep(h, prims, ext, num_templates, tmpl_h, tmpl_w, st))) return rc;
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(256), 0, st, h->counts, N, h->offs, dtot);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, h->counts, N, h->offs, dtot);
   LAUNCH_CHECK("k_scan_counts");
 
   int bits = 0;
@@ -480,6 +480,52 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd, dim3(h->tiles * h->subs), dim3(kThreads), 0, st, A,
                     grad_color, grad_alpha, out_color, out_T_final, out_last, h->grad8);
   LAUNCH_CHECK("k_raster_bwd");
+  if (N > 0) {
+    hipLaunchKernelGGL(k_unpack_grads, dim3(ceil_div(N, 256)), dim3(256), 0, st, h->grad8, N,
+                       grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);
+    LAUNCH_CHECK("k_unpack_grads");
+  }
+  timing_end(h, h->ev_bwd, st);
+  return 0;
+}
+
+int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                      const dbmp_tile_lists* lists, float eps_T, const float* target,
+                      const float* background, float background_value, float* loss_accum,
+                      const float* out_color, const float* out_T_final, const int32_t* out_last,
+                      float* grad_means2D, float* grad_radii, float* grad_rotations,
+                      float* grad_opacities, float* grad_colors, void* stream) {
+  (void)eps_T;
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  int rc = check_prims(prims);
+  if (rc) return rc;
+  if ((rc = check_bitmaps(bm))) return rc;
+  if (!lists || !lists->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  if (!target || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
+  if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
+    return fail(DBMP_E_ARG, "NULL gradient output");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = prims->num_primitives;
+  if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
+  timing_begin(h, h->ev_bwd, st);
+  if ((rc = ensure_quads(h, bm, st))) return rc;
+  if ((rc = ensure_cquads(h, bm, st))) return rc;
+  if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
+  if (N > 0) {
+    hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
+    LAUNCH_CHECK("k_fill_f32");
+  }
+  const RasterArgs A = make_args(h, prims, bm, lists, 0.f);
+  LossArgs L;
+  L.target = target;
+  L.bg = background;
+  L.bg_value = background_value;
+  L.inv_count = 1.f / (float)(3.0 * h->H * h->W);
+  L.loss = loss_accum;
+  const bool blend = A.cmap != nullptr;
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd_mse, dim3(h->tiles * h->subs), dim3(kThreads), 0, st,
+                    A, L, out_color, out_T_final, out_last, h->grad8);
+  LAUNCH_CHECK("k_raster_bwd_mse");
   if (N > 0) {
     hipLaunchKernelGGL(k_unpack_grads, dim3(ceil_div(N, 256)), dim3(256), 0, st, h->grad8, N,
                        grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);

@@ -385,24 +385,22 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   const float tnzf = okpix ? Tfin[pix] : 0.f;   // product of the nonzero (1 - a) factors
   const float tf = nzf ? 0.f : tnzf;            // transmittance after the last layer
   float gc0, gc1, gc2, ga;
+  float acc_loss = 0.f;   // this lane's squared error (LOSS)
   if constexpr (LOSS) {
     // the forward's alpha is 1 - T_final exactly
     const float om = 1.f - (1.f - tf);
     const float cf[3] = {cf0, cf1, cf2};
-    float g[3], acc = 0.f;
+    float g[3];
     ga = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float bgc = L.bg ? (okpix ? L.bg[3 * pix + c] : 0.f) : L.bg_value;
       const float r = okpix ? fmaf(om, bgc, cf[c]) - L.target[3 * pix + c] : 0.f;
-      acc = fmaf(r, r, acc);
+      acc_loss = fmaf(r, r, acc_loss);
       g[c] = 2.f * r * L.inv_count;
       ga = fmaf(-g[c], bgc, ga);
     }
     gc0 = g[0]; gc1 = g[1]; gc2 = g[2];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (G.lane == 0 && L.loss) unsafeAtomicAdd(L.loss, acc * L.inv_count);
   } else {
     gc0 = okpix ? gCin[3 * pix] : 0.f;
     gc1 = okpix ? gCin[3 * pix + 1] : 0.f;
@@ -517,6 +515,29 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     }
     __syncthreads();
   }
+  if constexpr (LOSS) {
+    // one loss atomic per workgroup, issued when its list walk is done (the
+    // workgroups finish at different times, so the same-address atomics do
+    // not arrive together)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc_loss += __shfl_xor(acc_loss, o, 64);
+    if (G.lane == 0) sred[0][0][G.w] = acc_loss;
+    __syncthreads();
+    if (threadIdx.x == 0 && L.loss)
+      unsafeAtomicAdd(L.loss, (sred[0][0][0] + sred[0][0][1] + sred[0][0][2] + sred[0][0][3]) *
+                                  L.inv_count);
+  }
+}
+
+// Backward with the background composite + MSE as its per-pixel prologue.
+template <int DT, bool BLEND>
+static __global__ __launch_bounds__(kThreads) void k_raster_bwd_mse(RasterArgs A, LossArgs L,
+                                                                    const float* __restrict__ Cfin,
+                                                                    const float* __restrict__ Tfin,
+                                                                    const int* __restrict__ lastin,
+                                                                    float* __restrict__ grad8) {
+  raster_bwd_block<DT, BLEND, true>(A, xcd_remap(blockIdx.x, A.tiles * A.subs), nullptr, nullptr, L,
+                                    Cfin, Tfin, lastin, grad8);
 }
 
 template <int DT, bool BLEND>

@@ -76,8 +76,6 @@ class FitJob:
         self.alpha = torch.empty((H, W), dtype=f32, device=dev)
         self.T = torch.empty((H, W), dtype=f32, device=dev)
         self.last = torch.empty((H, W), dtype=torch.int32, device=dev)
-        self.gC = torch.empty((H, W, 3), dtype=f32, device=dev)
-        self.gA = torch.empty((H, W), dtype=f32, device=dev)
         self.losses = torch.zeros(max_steps, dtype=f32, device=dev)
         self.t = 0
         lrs = lr if isinstance(lr, dict) else {k: float(lr) for k in GROUPS}
@@ -118,11 +116,11 @@ class FitJob:
         lists = h.bin_lazy(self._prims(False), self.ext, tuple(self.templates.shape))
         self.lists = lists
         h.forward(self._prims(True), self._bm, lists, self.eps_T, self.color, self.alpha, self.T, self.last)
-        h.loss_mse(self.color, self.alpha, self.target, self.gC, self.gA,
-                   self.losses[self.t:self.t + 1], bg=self.bg, bg_value=1.0)
         g = self.grads
-        h.backward(self._prims(True), self._bm, lists, self.eps_T, self.gC, self.gA, self.color, self.T,
-                   self.last, g["means2D"], g["radii"], g["rotations"], g["opacities"], g["colors"])
+        # background composite + MSE + its gradient fused into the backward pass
+        h.backward_mse(self._prims(True), self._bm, lists, self.eps_T, self.target, self.color, self.T,
+                       self.last, g["means2D"], g["radii"], g["rotations"], g["opacities"], g["colors"],
+                       loss_accum=self.losses[self.t:self.t + 1], bg=self.bg, bg_value=1.0)
         self.t += 1
         self.adam.step = self.t
         h.adam_step(self.adam, self.device)

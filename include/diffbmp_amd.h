@@ -178,6 +178,20 @@ typedef struct {
   int32_t store_dtype;
 } dbmp_adam;
 
+/* dbmp_loss_mse followed by dbmp_backward in one raster pass: the
+ * background composite, squared error and its gradient are computed per pixel
+ * in the backward's prologue, so no (H,W,3) gradient image is written or
+ * read.  Replaces the reference's torch composite + F.mse_loss + autograd
+ * feeding rasterize_tiles_backward (simple_tile_renderer.py:703-707,
+ * loss_functions.py:20-40, ext.cpp:31-50).  loss_accum (device, may be NULL)
+ * receives += the MSE. */
+int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                      const dbmp_tile_lists* lists, float eps_T, const float* target,
+                      const float* background, float background_value, float* loss_accum,
+                      const float* out_color, const float* out_T_final, const int32_t* out_last,
+                      float* grad_means2D, float* grad_radii, float* grad_rotations,
+                      float* grad_opacities, float* grad_colors, void* stream);
+
 /* torch.optim.Adam update (no weight decay, no amsgrad) of up to five groups
  * plus the reference's clamp x in [0,W-1], y in [0,H-1], r in [2, min(H,W)/4]
  * (simple_tile_renderer.py:122-128, 961-968, 1125-1131), one kernel. */

@@ -33,6 +33,35 @@ def test_loss_kernel_matches_autograd(dev):
         assert torch.allclose(gA, ga, rtol=1e-5, atol=1e-10)
 
 
+@pytest.mark.parametrize("with_bg", [False, True])
+def test_fused_backward_mse_matches_two_kernels(dev, with_bg):
+    """dbmp_backward_mse equals dbmp_loss_mse followed by dbmp_backward."""
+    from helpers import GpuRun
+    sc = S.make_scene(0, with_target=False)
+    run = GpuRun(sc, dev, eps_T=1e-6)
+    run.forward()
+    g = torch.Generator(dev).manual_seed(7)
+    target = torch.rand((sc.H, sc.W, 3), device=dev, generator=g)
+    bg = torch.rand((sc.H, sc.W, 3), device=dev, generator=g) if with_bg else None
+    gC = torch.empty_like(target)
+    gA = torch.empty((sc.H, sc.W), device=dev)
+    l1 = torch.zeros(1, device=dev)
+    run.h.loss_mse(run.color, run.alpha, target, gC, gA, l1, bg=bg)
+    ref = run.backward(gC.cpu().numpy(), gA.cpu().numpy())
+    N = sc.N
+    out = [torch.empty(sh, device=dev) for sh in ((N, 2), (N,), (N,), (N,), (N, 3))]
+    l2 = torch.zeros(1, device=dev)
+    from diffbmp_amd import _native
+    run.h.backward_mse(run.prims, run.bm, _native.Lists(run.offs, run.idx), run.eps, target, run.color,
+                       run.T, run.last, *out, loss_accum=l2, bg=bg)
+    torch.cuda.synchronize()
+    assert abs(l1.item() - l2.item()) <= 1e-5 * l1.item()   # fp32 sums in another order
+    for k, o in zip(("means2D", "radii", "rotations", "opacities", "colors"), out):
+        o = o.cpu().numpy()
+        err = np.linalg.norm(o - ref[k]) / np.linalg.norm(ref[k])
+        assert err <= 1e-6, (k, err)
+
+
 def test_fitjob_tracks_autograd_adam(dev):
     sc = S.make_scene(0, with_target=False)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
