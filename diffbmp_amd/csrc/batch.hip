@@ -80,8 +80,8 @@ int check_job(const dbmp_job& j, int i) {
   if (j.num_primitives < 0) return fail(DBMP_E_ARG, "job %d: num_primitives < 0", i);
   if (j.num_templates <= 0 || j.tmpl_height < 2 || j.tmpl_width < 2)
     return fail(DBMP_E_ARG, "job %d: bad template shape", i);
-  if ((long long)j.num_templates * j.tmpl_height * j.tmpl_width >= (1LL << 31))
-    return fail(DBMP_E_ARG, "job %d: templates hold more than 2^31 texels", i);
+  if ((long long)j.num_templates * j.tmpl_height * j.tmpl_width >= (1LL << 28))
+    return fail(DBMP_E_ARG, "job %d: templates hold more than 2^28 texels", i);
   if (!j.quads || !j.ext) return fail(DBMP_E_ARG, "job %d: prepared bitmaps missing", i);
   if (j.num_primitives > 0 && (!j.bmp_sel || !j.params || !j.exp_avg || !j.exp_avg_sq || !j.store))
     return fail(DBMP_E_ARG, "job %d: a parameter pointer is NULL", i);
@@ -274,8 +274,8 @@ int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext
   if (bm->num_templates <= 0 || bm->height < 2 || bm->width < 2)
     return fail(DBMP_E_ARG, "templates must be (T>0, H>=2, W>=2)");
   if (bm->dtype != b->dtype) return fail(DBMP_E_ARG, "template dtype differs from the batch's");
-  if ((long long)bm->num_templates * bm->height * bm->width >= (1LL << 31))
-    return fail(DBMP_E_ARG, "templates hold more than 2^31 texels");
+  if ((long long)bm->num_templates * bm->height * bm->width >= (1LL << 28))
+    return fail(DBMP_E_ARG, "templates hold more than 2^28 texels");
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(bm->num_templates), block(256);
   if (bm->dtype == DBMP_F32)

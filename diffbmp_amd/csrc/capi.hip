@@ -130,8 +130,8 @@ int check_bitmaps(const dbmp_bitmaps* b) {
     return fail(DBMP_E_ARG, "templates must be (T>0, H>=2, W>=2), got (%d,%d,%d)",
                 b->num_templates, b->height, b->width);
   if (b->dtype < DBMP_F32 || b->dtype > DBMP_BF16) return fail(DBMP_E_ARG, "bad bitmap dtype %d", b->dtype);
-  if ((long long)b->num_templates * b->height * b->width >= (1LL << 31))
-    return fail(DBMP_E_ARG, "templates hold more than 2^31 texels");
+  if ((long long)b->num_templates * b->height * b->width >= (1LL << 28))
+    return fail(DBMP_E_ARG, "templates hold more than 2^28 texels");
   if (b->colors_orig_mode < 0 || b->colors_orig_mode > 2)
     return fail(DBMP_E_ARG, "bad colors_orig_mode %d", b->colors_orig_mode);
   if (b->colors_orig_mode != 0 && !b->colors_orig) return fail(DBMP_E_ARG, "colors_orig is NULL");

@@ -48,22 +48,23 @@ __device__ __forceinline__ float f16_hi(uint32_t w) {
 template <int DT> struct QuadT;
 template <> struct QuadT<kF32> {
   using type = float4;
-  __device__ static __forceinline__ void get(const float4* q, long i, float& a, float& b, float& c, float& d) {
-    float4 v = q[i];
+  // i < 2^28 (the byte offset fits 32 bits: one base register + a 32-bit offset)
+  __device__ static __forceinline__ void get(const float4* q, unsigned i, float& a, float& b, float& c, float& d) {
+    const float4 v = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q) + (i << 4));
     a = v.x; b = v.y; c = v.z; d = v.w;
   }
 };
 template <> struct QuadT<kF16> {
   using type = uint2;
-  __device__ static __forceinline__ void get(const uint2* q, long i, float& a, float& b, float& c, float& d) {
-    uint2 v = q[i];
+  __device__ static __forceinline__ void get(const uint2* q, unsigned i, float& a, float& b, float& c, float& d) {
+    const uint2 v = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(q) + (i << 3));
     a = f16_lo(v.x); b = f16_hi(v.x); c = f16_lo(v.y); d = f16_hi(v.y);
   }
 };
 template <> struct QuadT<kBF16> {
   using type = uint2;
-  __device__ static __forceinline__ void get(const uint2* q, long i, float& a, float& b, float& c, float& d) {
-    uint2 v = q[i];
+  __device__ static __forceinline__ void get(const uint2* q, unsigned i, float& a, float& b, float& c, float& d) {
+    const uint2 v = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(q) + (i << 3));
     a = bf16_lo(v.x); b = bf16_hi(v.x); c = bf16_lo(v.y); d = bf16_hi(v.y);
   }
 };

@@ -245,8 +245,8 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   // that is not `ok` reads some valid quad and its sample is discarded
   // texel index within the template in float (exact below 2^24), clamped so
   // that NaN or out-of-range coordinates still address a texel of the template
-  const float fl = fminf(fmaxf(fmaf(fy, A.Wtm1 + 1.f, fx), 0.f), A.HWm1f);
-  const int idx = p.tbase + (int)fl;
+  const float fl = __builtin_amdgcn_fmed3f(fmaf(fy, A.Wtm1 + 1.f, fx), 0.f, A.HWm1f);
+  const unsigned idx = (unsigned)(p.tbase + (int)fl);
   float v00, v01, v10, v11;
   Q::get((const typename Q::type*)A.quads, idx, v00, v01, v10, v11);
   const float wx = r.tx - fx, wy = r.ty - fy;
@@ -257,7 +257,7 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
     m = bilerp(v00, v01, v10, v11, wx, wy);
     r.gxt = r.gyt = 0.f;
   }
-  r.m = fmaxf(0.f, fminf(1.f, m));
+  r.m = __builtin_amdgcn_fmed3f(m, 0.f, 1.f);   // clamp to [0, 1]
   r.pushed = ok & (r.m > 1e-6f);
   return r;
 }
@@ -311,12 +311,12 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         const float f = fmaxf(1.f - a, 0.f);
         T = T * f;
         Tnz = f > 0.f ? Tnz * f : Tnz;
-        nz = f > 0.f ? nz : min(nz + 1, 2);
+        nz = f > 0.f ? nz : min(nz + 1, 2);   // exactly opaque layers, capped at 2
         // a pixel stops once later layers can change neither its colour nor
         // its alpha (or alpha's derivative w.r.t. an opaque layer) by eps_T
-        const bool dead = S.pushed && (Tnz < A.eps || nz >= 2);
+        const bool dead = S.pushed & ((Tnz < A.eps) | (nz >= 2));
         last = dead ? base - start + j + 1 : last;
-        alive = alive && !dead;
+        alive = alive & !dead;
         if (!__any(alive)) break;
       }
     }
