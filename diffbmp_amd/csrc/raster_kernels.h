@@ -464,18 +464,19 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       dLda = fmaf(gc0, fmaf(Tk, col[0], -S0 * inv1m), dLda);
       dLda = fmaf(gc1, fmaf(Tk, col[1], -S1 * inv1m), dLda);
       dLda = fmaf(gc2, fmaf(Tk, col[2], -S2 * inv1m), dLda);
+      // Per-sample parts of the 8 gradients.  Factors that are constant per
+      // primitive (-1/r, dalpha/dv, sigma'(c), the blend weight) are applied
+      // once per (workgroup, entry) when the wave partials are summed below.
       float acc[8];
-      // colour logits (tile_backward.cu:255-261); wgt = 0 for lanes not pushed.
-      // With an exact colour-map gradient the blend weight (1 - c_blend)
-      // applies; the reference leaves it out.
       const bool exact = has_map && A.exact_blend;
-      const float cw = exact ? wgt * (1.f - A.c_blend) : wgt;
-      acc[5] = gc0 * cw * (p.cr * (1.f - p.cr));
-      acc[6] = gc1 * cw * (p.cg * (1.f - p.cg));
-      acc[7] = gc2 * cw * (p.cb * (1.f - p.cb));
+      // colour logits (tile_backward.cu:255-261): dC_c/dcol = T a; wgt = 0 for
+      // lanes not pushed
+      acc[5] = gc0 * wgt;
+      acc[6] = gc1 * wgt;
+      acc[7] = gc2 * wgt;
       // opacity logit (:311-312): alpha_ub * mask * s (1 - s)
       const float me = S.pushed ? S.m : 0.f;
-      acc[4] = dLda * (me * p.dav);
+      acc[4] = dLda * me;
       // mask path (:314-343)
       const float dLdm = S.pushed ? dLda * p.abase : 0.f;
       float dLdu = dLdm * (S.gxt * A.hsx);
@@ -488,13 +489,15 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
           dLdv = fmaf(kw * A.hsy, fmaf(gc0, dcy[0], fmaf(gc1, dcy[1], gc2 * dcy[2])), dLdv);
         }
       }
-      const float pc = p.c * p.rinv, ps = p.s * p.rinv;
-      acc[0] = fmaf(dLdu, -pc, dLdv * ps);
-      acc[1] = fmaf(dLdu, -ps, -dLdv * pc);
-      acc[2] = -(fmaf(dLdu, S.u, dLdv * S.v)) * p.rinv;
-      const float dudp = fmaf(-p.s, S.ndx, p.c * S.ndy);
-      const float dvdp = fmaf(-p.c, S.ndx, -p.s * S.ndy);
-      acc[3] = fmaf(dLdu, dudp, dLdv * dvdp);
+      // (u, v) = R(theta) (dx, dy) / r: with (gx, gy) = R^T (dL/du, dL/dv),
+      //   dL/dmx = -gx/r, dL/dmy = -gy/r, dL/dr = -(ndx gx + ndy gy)/r,
+      //   dL/dtheta = ndy gx - ndx gy
+      const float gx = fmaf(p.c, dLdu, -p.s * dLdv);
+      const float gy = fmaf(p.s, dLdu, p.c * dLdv);
+      acc[0] = gx;
+      acc[1] = gy;
+      acc[2] = fmaf(S.ndx, gx, S.ndy * gy);
+      acc[3] = fmaf(S.ndy, gx, -S.ndx * gy);
       T = Tn;
       int which;
       const float tot = wave_reduce_scatter8(acc, which);
@@ -502,10 +505,15 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     }
     if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G.bx, G.by);
     __syncthreads();
-    // one atomic per (tile, entry, value): sum the four waves' partials
+    // one atomic per (tile, entry, value): sum the four waves' partials and
+    // apply the per-primitive factor of value q
+    const float cwf = (BLEND && A.exact_blend && A.cmap != nullptr) ? 1.f - A.c_blend : 1.f;
     for (int i = threadIdx.x; i < cnt * 8; i += kThreads) {
       const int j = i >> 3, q = i & 7;
-      const float s = sred[0][j][q] + sred[1][j][q] + sred[2][j][q] + sred[3][j][q];
+      const PrimRec& pr = srec[buf][j];
+      const float cc = q == 5 ? pr.cr : (q == 6 ? pr.cg : pr.cb);
+      const float fac = q < 3 ? -pr.rinv : (q == 3 ? 1.f : (q == 4 ? pr.dav : cc * (1.f - cc) * cwf));
+      const float s = (sred[0][j][q] + sred[1][j][q] + sred[2][j][q] + sred[3][j][q]) * fac;
       sred[0][j][q] = 0.f;
       sred[1][j][q] = 0.f;
       sred[2][j][q] = 0.f;
