@@ -291,10 +291,12 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     const bool have_next = base + kBatch < end;
     Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
-    const int cnt = __builtin_amdgcn_readfirstlane(min(kBatch, end - base));
     if (__any(alive)) {
-      for (int j = 0; j < cnt; ++j) {
-        if (!(__builtin_amdgcn_readfirstlane(smask[buf][j]) & wbit)) continue;
+      // entries of this batch whose push box can touch this wave's rows
+      unsigned long long wm = __ballot((smask[buf][G.lane] & wbit) != 0);
+      while (wm) {
+        const int j = __builtin_ctzll(wm);
+        wm &= wm - 1ull;
         const PrimRec p = srec[buf][j];
         const Sample<DT> S = eval_sample<DT, false>(A, p, fpx, fpy, alive);
         if constexpr (BLEND) {
@@ -426,10 +428,15 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
     const int cnt = __builtin_amdgcn_readfirstlane(min(kBatch, stop - base));
-    for (int j = 0; j < cnt; ++j) {
+    // entries of this batch that touch this wave's rows, up to the wave's
+    // furthest forward position
+    unsigned long long wm = __ballot((smask[buf][G.lane] & wbit) != 0);
+    const int lim = wkmax - (base - start);
+    wm = lim >= 64 ? wm : (lim <= 0 ? 0ull : wm & ((1ull << lim) - 1ull));
+    while (wm) {
+      const int j = __builtin_ctzll(wm);
+      wm &= wm - 1ull;
       const int kk = base - start + j;
-      if (kk >= wkmax) break;
-      if (!(__builtin_amdgcn_readfirstlane(smask[buf][j]) & wbit)) continue;
       const PrimRec p = srec[buf][j];
       const Sample<DT> S = eval_sample<DT, true>(A, p, fpx, fpy, kk < lst);
       if (!__any(S.pushed)) continue;
