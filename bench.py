@@ -49,9 +49,10 @@ def parse():
     ap.add_argument("--psnr-steps", type=int, default=0,
                     help="also fit this many Adam steps at fp32 and at --precision from the same "
                          "start and report both final PSNRs")
-    ap.add_argument("--workload", default="scale", choices=["scale", "serving"],
-                    help="scale: the headline fit step; serving: a Poisson stream of fit jobs "
-                         "through the batched multi-job engine (its own JSON line)")
+    ap.add_argument("--workload", default="scale", choices=["scale", "stress", "serving"],
+                    help="scale: the headline fit step; stress: the 2048^2 / 32768-primitive "
+                         "RGBA colour-mapped fit step; serving: a Poisson stream of fit jobs "
+                         "through the batched multi-job engine (each its own JSON line)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="arrival rate in jobs/s (0: every job arrives at t = 0, i.e. capacity)")
@@ -222,6 +223,56 @@ def list_stats(handle, sc, dev, sdt, eps_T):
     return int(offs[-1].item()), float(lens.mean().item()), float(walked)
 
 
+def run_stress(args, dev):
+    """Fused fit step on the stress workload: 2048^2 canvas, 32768 primitives,
+    16 RGBA 256^2 bitmaps (alpha template + per-template colour map) blended
+    with c_blend 0.5, storage in --precision (fp16 by default here)."""
+    import torch
+    from diffbmp_amd import scenes as S
+    from diffbmp_amd.fit import FitJob
+
+    sc = S.make_scene(3)
+    sdt = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[args.precision]
+    pr = make_problem(sc, dev, sdt)
+    co = torch.from_numpy(np.ascontiguousarray(sc.colors_orig_tmpl)).to(dev)
+    job = FitJob(*pr["params"], pr["templates"], pr["sel"], pr["target"], lr=1e-2, store_dtype=sdt,
+                 max_steps=args.warmup + args.steps + 8, colors_orig=co, c_blend=sc.c_blend,
+                 colors_orig_per_template=True)
+    for _ in range(args.warmup):
+        job.step()
+    torch.cuda.synchronize()
+    job.handle.timing_reset()
+    job.handle.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        job.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    job.handle.timing_enable(False)
+    f_ms, f_n, b_ms, b_n = job.handle.timing_stats()
+    T, Ht, Wt = sc.templates.shape
+    out = {
+        "metric": "fit steps/s at 2048^2, 32768 prims, 16 RGBA 256^2 bitmaps",
+        "value": round(args.steps / (t1 - t0), 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round((t1 - t0) / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "dtype": {"fp32": "f32", "fp16": "f16-store/f32-acc", "bf16": "bf16-store/f32-acc"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "stress: 2048x2048 RGB, 32768 primitives, 16x256^2 RGBA bitmaps "
+                               "(per-template colour maps, c_blend 0.5), Adam lr 1e-2 on MSE",
+                   "canvas": [sc.H, sc.W], "primitives": sc.N, "bitmaps": [T, Ht, Wt],
+                   "precision": args.precision},
+        "raster_fwd_ms": round(f_ms / max(f_n, 1), 4),
+        "raster_bwd_ms": round(b_ms / max(b_n, 1), 4),
+        "final_loss": float(job.losses[job.t - 1].item()),
+    }
+    print(json.dumps(out))
+
+
 def run_serving(args, dev):
     """Poisson stream of fit jobs (canvas side uniform over {256, 512, 1024,
     2048}, primitives log-uniform in [512, 32768], 1-16 of the stress
@@ -295,6 +346,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.workload == "serving":
         run_serving(args, dev)
+        return
+    if args.workload == "stress":
+        run_stress(args, dev)
         return
     if args.copy_probe:
         nb = args.copy_probe << 20

@@ -65,7 +65,7 @@ class Job(ctypes.Structure):
                 ("num_primitives", ctypes.c_int32), ("num_templates", ctypes.c_int32),
                 ("tmpl_height", ctypes.c_int32), ("tmpl_width", ctypes.c_int32),
                 ("quads", ctypes.c_void_p), ("ext", ctypes.c_void_p),
-                ("color_quads", ctypes.c_void_p), ("c_blend", ctypes.c_float),
+                ("quads_with_color", ctypes.c_int32), ("c_blend", ctypes.c_float),
                 ("blend_grad_exact", ctypes.c_int32), ("bmp_sel", ctypes.c_void_p),
                 ("params", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
                 ("exp_avg_sq", ctypes.c_void_p), ("store", ctypes.c_void_p),
@@ -140,7 +140,7 @@ EXPORTS = {
     "dbmp_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "dbmp_batch_prepare_bitmaps": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Bitmaps),
                                                   ctypes.c_void_p, ctypes.c_void_p,
-                                                  ctypes.c_void_p, ctypes.c_void_p]),
+                                                  ctypes.c_void_p]),
     "dbmp_batch_set_jobs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Job), ctypes.c_int32,
                                            ctypes.c_int64, ctypes.c_void_p]),
     "dbmp_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),

@@ -267,7 +267,7 @@ int dbmp_batch_destroy(dbmp_batch* b) {
 }
 
 int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext_out,
-                               void* quads_out, void* color_quads_out, void* stream) {
+                               void* quads_out, void* stream) {
   if (!b) return fail(DBMP_E_ARG, "batch is NULL");
   if (!bm || !bm->templates) return fail(DBMP_E_ARG, "templates are NULL");
   if (!ext_out || !quads_out) return fail(DBMP_E_ARG, "NULL output");
@@ -288,13 +288,9 @@ int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext
     hipLaunchKernelGGL(k_template_extents<__hip_bfloat16>, grid, block, 0, st,
                        (const __hip_bfloat16*)bm->templates, bm->height, bm->width, ext_out);
   LAUNCH_CHECK("k_template_extents");
-  int rc = launch_build_quads(bm, quads_out, st);
-  if (rc) return rc;
-  if (bm->colors_orig_mode == 2 && bm->colors_orig) {
-    if (!color_quads_out) return fail(DBMP_E_ARG, "color_quads_out is NULL");
-    return launch_build_cquads(bm, color_quads_out, st);
-  }
-  return 0;
+  if (bm->colors_orig_mode == 2 && bm->colors_orig) return launch_build_rgba(bm, quads_out, st);
+  if (bm->colors_orig_mode != 0) return fail(DBMP_E_ARG, "colour maps must be per template (mode 2)");
+  return launch_build_quads(bm, quads_out, st);
 }
 
 int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
@@ -350,10 +346,11 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.pcount = b->dtot;
     A.pcap = pair_capacity;
     A.quads = s.quads;
-    const bool bl = s.color_quads != nullptr && s.c_blend > 0.f;
-    blend = blend || bl;
-    A.cmap = bl ? s.color_quads : nullptr;
-    A.cmap_mode = 2;
+    // RGBA records need the blend kernels (their stride), blending or not
+    const bool rec = s.quads_with_color != 0;
+    blend = blend || rec;
+    A.cmap = (rec && s.c_blend > 0.f) ? s.quads : nullptr;
+    A.cmap_mode = rec ? 2 : 1;
     A.c_blend = s.c_blend;
     A.exact_blend = s.blend_grad_exact != 0;
     A.N = s.num_primitives;

@@ -71,12 +71,10 @@ struct dbmp_rasterizer {
   const void* q_src = nullptr;
   long long q_version = -2;
   int q_T = 0, q_Ht = 0, q_Wt = 0, q_dt = -1;
-  // colour-map quads cache (per-template maps)
-  void* cquads = nullptr;
-  size_t cquads_bytes = 0;
-  const void* cq_src = nullptr;
-  long long cq_version = -2;
-  int cq_T = 0, cq_Ht = 0, cq_Wt = 0, cq_dt = -1;
+  // the quads hold RGBA records (alpha + per-template colour maps)
+  bool q_rgba = false;
+  const void* q_csrc = nullptr;
+  long long q_cversion = -2;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_fwd, ev_bwd;
@@ -160,12 +158,22 @@ int launch_prep(dbmp_rasterizer* h, const dbmp_prims* p, const float* ext, int T
   return 0;
 }
 
+// Texel quads of the templates, or RGBA records when the call blends
+// per-template colour maps (colors_orig_mode 2, c_blend > 0).  Rebuilt when
+// the source pointers, versions (-1: every call), shape, type or kind change.
+bool wants_rgba(const dbmp_bitmaps* b) {
+  return b->colors_orig_mode == 2 && b->colors_orig != nullptr && b->c_blend > 0.f;
+}
+
 int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   const long long tot = (long long)b->num_templates * b->height * b->width;
-  const size_t esz = b->dtype == DBMP_F32 ? 16 : 8;
+  const bool rgba = wants_rgba(b);
+  const size_t esz = rgba ? rgba_bytes(b->dtype) : quad_bytes(b->dtype);
   if (b->version >= 0 && h->q_src == b->templates && h->q_version == b->version &&
       h->q_T == b->num_templates && h->q_Ht == b->height && h->q_Wt == b->width &&
-      h->q_dt == b->dtype)
+      h->q_dt == b->dtype && h->q_rgba == rgba &&
+      (!rgba || (b->colors_version >= 0 && h->q_csrc == b->colors_orig &&
+                 h->q_cversion == b->colors_version)))
     return 0;
   if ((size_t)tot * esz > h->quads_bytes) {
     if (h->quads) (void)hipFree(h->quads);
@@ -175,7 +183,7 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
     if (e != hipSuccess) return fail(DBMP_E_ALLOC, "hipMalloc quads: %s", hipGetErrorString(e));
     h->quads_bytes = (size_t)tot * esz;
   }
-  int rc = launch_build_quads(b, h->quads, st);
+  int rc = rgba ? launch_build_rgba(b, h->quads, st) : launch_build_quads(b, h->quads, st);
   if (rc) return rc;
   h->q_src = b->templates;
   h->q_version = b->version;
@@ -183,35 +191,9 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   h->q_Ht = b->height;
   h->q_Wt = b->width;
   h->q_dt = b->dtype;
-  return 0;
-}
-
-// Colour-map quads of per-template colour maps (mode 2), rebuilt when the
-// maps' pointer, version (-1: every call), shape or type changes.
-int ensure_cquads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
-  if (b->colors_orig_mode != 2 || !(b->c_blend > 0.f)) return 0;
-  const long long tot = (long long)b->num_templates * b->height * b->width;
-  const size_t bytes = (size_t)tot * cquad_bytes(b->dtype);
-  if (b->colors_version >= 0 && h->cq_src == b->colors_orig && h->cq_version == b->colors_version &&
-      h->cq_T == b->num_templates && h->cq_Ht == b->height && h->cq_Wt == b->width &&
-      h->cq_dt == b->dtype)
-    return 0;
-  if (bytes > h->cquads_bytes) {
-    if (h->cquads) (void)hipFree(h->cquads);
-    h->cquads = nullptr;
-    h->cquads_bytes = 0;
-    hipError_t e = hipMalloc(&h->cquads, bytes);
-    if (e != hipSuccess) return fail(DBMP_E_ALLOC, "hipMalloc colour quads: %s", hipGetErrorString(e));
-    h->cquads_bytes = bytes;
-  }
-  int rc = launch_build_cquads(b, h->cquads, st);
-  if (rc) return rc;
-  h->cq_src = b->colors_orig;
-  h->cq_version = b->colors_version;
-  h->cq_T = b->num_templates;
-  h->cq_Ht = b->height;
-  h->cq_Wt = b->width;
-  h->cq_dt = b->dtype;
+  h->q_rgba = rgba;
+  h->q_csrc = b->colors_orig;
+  h->q_cversion = b->colors_version;
   return 0;
 }
 
@@ -226,8 +208,9 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.quads = h->quads;
   const bool blend = b->colors_orig_mode != 0 && b->c_blend > 0.f;
   // per-template maps are sampled through their colour-map quads
-  A.cmap = !blend ? nullptr : (b->colors_orig_mode == 2 ? h->cquads : b->colors_orig);
-  A.cmap_mode = b->colors_orig_mode == 2 ? 2 : 1;
+  // per-template maps are sampled from the RGBA records in `quads`
+  A.cmap = !blend ? nullptr : (b->colors_orig_mode == 2 ? h->quads : b->colors_orig);
+  A.cmap_mode = (blend && b->colors_orig_mode == 2) ? 2 : 1;
   A.c_blend = b->c_blend;
   A.exact_blend = b->blend_grad_exact != 0;
   A.N = p->num_primitives;
@@ -327,7 +310,7 @@ int dbmp_destroy(dbmp_rasterizer* h) {
   timing_drain(h->ev_fwd, ms, n);
   timing_drain(h->ev_bwd, ms, n);
   void* ptrs[] = {h->recs, h->counts, h->offs, h->ranges, h->grad8, h->keysA,
-                  h->keysB, h->valsB, h->hist, h->totals, h->quads, h->cquads};
+                  h->keysB, h->valsB, h->hist, h->totals, h->quads};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -438,7 +421,6 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
   timing_begin(h, h->ev_fwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
-  if ((rc = ensure_cquads(h, bm, st))) return rc;
   // raster constants only (no extents => no tile ranges)
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
   const RasterArgs A = make_args(h, prims, bm, lists, eps_T);
@@ -469,7 +451,6 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
   timing_begin(h, h->ev_bwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
-  if ((rc = ensure_cquads(h, bm, st))) return rc;
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
   if (N > 0) {
     hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
@@ -509,7 +490,6 @@ int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bi
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
   timing_begin(h, h->ev_bwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
-  if ((rc = ensure_cquads(h, bm, st))) return rc;
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
   if (N > 0) {
     hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);

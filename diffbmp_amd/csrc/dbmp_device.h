@@ -69,38 +69,44 @@ template <> struct QuadT<kBF16> {
   }
 };
 
-// Colour-map quads: entry (m, y, x) holds the RGB of texels (x, y), (x1, y),
-// (x, y1), (x1, y1) with x1 = min(x+1, W-1), y1 = min(y+1, H-1) (clamp to
-// edge), as 12 values padded to 16: one aligned 64-byte (fp32) or 32-byte
-// (fp16 / bf16) record per bilinear colour sample.
-template <int DT> struct CQuad;
-template <> struct CQuad<kF32> {
-  using type = float4;
-  __device__ static __forceinline__ void get(const float4* q, long i, float c00[3], float c10[3],
-                                             float c01[3], float c11[3]) {
-    const float4 a = q[4 * i], b = q[4 * i + 1], c = q[4 * i + 2];
-    c00[0] = a.x; c00[1] = a.y; c00[2] = a.z;
-    c10[0] = a.w; c10[1] = b.x; c10[2] = b.y;
-    c01[0] = b.z; c01[1] = b.w; c01[2] = c.x;
-    c11[0] = c.y; c11[1] = c.z; c11[2] = c.w;
+// RGBA texel records of a template set with per-template colour maps: entry
+// (t, y, x) holds the template's bilinear neighbourhood (the alpha quad of
+// QuadT, zero outside the bitmap) followed by the colour map's clamp-to-edge
+// neighbourhood: RGB of (x,y), (x1,y), (x,y1), (x1,y1), x1 = min(x+1, W-1),
+// y1 = min(y+1, H-1).  16 values, one aligned 64-byte (fp32) or 32-byte
+// (fp16 / bf16) record: a sample's alpha and colour arrive in one cache line.
+template <int DT> struct RecT;
+template <> struct RecT<kF32> {
+  struct Col { float v[12]; };
+  __device__ static __forceinline__ void get(const void* base, unsigned i, float& a, float& b,
+                                             float& c, float& d, Col& col) {
+    const float4* q = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (i << 6));
+    const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
+    a = r0.x; b = r0.y; c = r0.z; d = r0.w;
+    col.v[0] = r1.x; col.v[1] = r1.y; col.v[2] = r1.z; col.v[3] = r1.w;
+    col.v[4] = r2.x; col.v[5] = r2.y; col.v[6] = r2.z; col.v[7] = r2.w;
+    col.v[8] = r3.x; col.v[9] = r3.y; col.v[10] = r3.z; col.v[11] = r3.w;
   }
+  __device__ static __forceinline__ float at(const Col& col, int k) { return col.v[k]; }
 };
 template <int DT>
-struct CQuadHalf {
-  using type = uint4;
+struct RecHalf {
+  struct Col { uint32_t v[6]; };
   __device__ static __forceinline__ float lo(uint32_t v) { return DT == kF16 ? f16_lo(v) : bf16_lo(v); }
   __device__ static __forceinline__ float hi(uint32_t v) { return DT == kF16 ? f16_hi(v) : bf16_hi(v); }
-  __device__ static __forceinline__ void get(const uint4* q, long i, float c00[3], float c10[3],
-                                             float c01[3], float c11[3]) {
-    const uint4 a = q[2 * i], b = q[2 * i + 1];
-    c00[0] = lo(a.x); c00[1] = hi(a.x); c00[2] = lo(a.y);
-    c10[0] = hi(a.y); c10[1] = lo(a.z); c10[2] = hi(a.z);
-    c01[0] = lo(a.w); c01[1] = hi(a.w); c01[2] = lo(b.x);
-    c11[0] = hi(b.x); c11[1] = lo(b.y); c11[2] = hi(b.y);
+  __device__ static __forceinline__ void get(const void* base, unsigned i, float& a, float& b,
+                                             float& c, float& d, Col& col) {
+    const uint4* q = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(base) + (i << 5));
+    const uint4 r0 = q[0], r1 = q[1];
+    a = lo(r0.x); b = hi(r0.x); c = lo(r0.y); d = hi(r0.y);
+    col.v[0] = r0.z; col.v[1] = r0.w; col.v[2] = r1.x; col.v[3] = r1.y; col.v[4] = r1.z; col.v[5] = r1.w;
+  }
+  __device__ static __forceinline__ float at(const Col& col, int k) {
+    return (k & 1) ? hi(col.v[k >> 1]) : lo(col.v[k >> 1]);
   }
 };
-template <> struct CQuad<kF16> : CQuadHalf<kF16> {};
-template <> struct CQuad<kBF16> : CQuadHalf<kBF16> {};
+template <> struct RecT<kF16> : RecHalf<kF16> {};
+template <> struct RecT<kBF16> : RecHalf<kBF16> {};
 
 // sigmoidf_safe (tile_common.h:125-133); evaluated once per primitive
 __device__ __forceinline__ float sigmoid_safe(float x) {

@@ -71,38 +71,48 @@ static __global__ void k_build_quads(const void* src, int T, int Ht, int Wt, voi
   }
 }
 
-// Colour-map quads (dbmp::CQuad) of M maps (M,Ht,Wt,3): entry (m,y,x) holds
-// the RGB of (x,y), (x1,y), (x,y1), (x1,y1), clamped to the map, padded to
-// 16 values.
+// RGBA texel records (dbmp::RecT) of T templates (T,Ht,Wt) and their colour
+// maps (T,Ht,Wt,3): per texel the alpha quad (zero outside the bitmap) and
+// the colour map's clamp-to-edge neighbourhood, 16 values.
 template <int DT>
-static __global__ void k_build_cquads(const void* src, long long M, int Ht, int Wt, void* dst) {
+static __global__ void k_build_rgba(const void* alpha, const void* color, int T, int Ht, int Wt,
+                                    void* dst) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M * Ht * Wt) return;
+  if (i >= (long long)T * Ht * Wt) return;
   const int x = (int)(i % Wt);
   const int y = (int)((i / Wt) % Ht);
-  const long long base = i - ((long long)y * Wt + x);   // first texel of map m
-  const int x1 = x + 1 < Wt ? x + 1 : Wt - 1, y1 = y + 1 < Ht ? y + 1 : Ht - 1;
+  const bool rx = x + 1 < Wt, ry = y + 1 < Ht;
+  const long long base = i - ((long long)y * Wt + x);
+  const int x1 = rx ? x + 1 : Wt - 1, y1 = ry ? y + 1 : Ht - 1;
   const long long t[4] = {base + (long long)y * Wt + x, base + (long long)y * Wt + x1,
                           base + (long long)y1 * Wt + x, base + (long long)y1 * Wt + x1};
   if constexpr (DT == kF32) {
-    const float* s = (const float*)src;
+    const float* a = (const float*)alpha;
+    const float* c = (const float*)color;
     float v[16];
+    v[0] = a[i];
+    v[1] = rx ? a[i + 1] : 0.f;
+    v[2] = ry ? a[i + Wt] : 0.f;
+    v[3] = (rx && ry) ? a[i + Wt + 1] : 0.f;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[3 * k + c] = s[3 * t[k] + c];
-    v[12] = v[13] = v[14] = v[15] = 0.f;
+      for (int ch = 0; ch < 3; ++ch) v[4 + 3 * k + ch] = c[3 * t[k] + ch];
     float4* d = (float4*)dst + 4 * i;
 #pragma unroll
     for (int k = 0; k < 4; ++k) d[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
   } else {
-    const unsigned short* s = (const unsigned short*)src;   // raw fp16 / bf16 bits
+    const unsigned short* a = (const unsigned short*)alpha;   // raw fp16 / bf16 bits
+    const unsigned short* c = (const unsigned short*)color;
     uint32_t v[16];
+    v[0] = a[i];
+    v[1] = rx ? a[i + 1] : 0u;
+    v[2] = ry ? a[i + Wt] : 0u;
+    v[3] = (rx && ry) ? a[i + Wt + 1] : 0u;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[3 * k + c] = s[3 * t[k] + c];
-    v[12] = v[13] = v[14] = v[15] = 0u;
+      for (int ch = 0; ch < 3; ++ch) v[4 + 3 * k + ch] = c[3 * t[k] + ch];
     uint4* d = (uint4*)dst + 2 * i;
     d[0] = make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16));
     d[1] = make_uint4(v[8] | (v[9] << 16), v[10] | (v[11] << 16), v[12] | (v[13] << 16),
@@ -142,22 +152,24 @@ inline int launch_build_quads(const dbmp_bitmaps* b, void* dst, hipStream_t st) 
   return 0;
 }
 
-// Colour-map quads of the per-template maps of `b` (T,Ht,Wt,3) into dst
-// (T*Ht*Wt records of 64 bytes for fp32, 32 bytes otherwise).
-inline int launch_build_cquads(const dbmp_bitmaps* b, void* dst, hipStream_t st) {
+// RGBA texel records of the templates and per-template colour maps of `b`
+// into dst (T*Ht*Wt records of rgba_bytes(dtype)).
+inline int launch_build_rgba(const dbmp_bitmaps* b, void* dst, hipStream_t st) {
   const long long tot = (long long)b->num_templates * b->height * b->width;
   if (tot == 0) return 0;
   const dim3 grid(ceil_div(tot, 256)), block(256);
   if (b->dtype == DBMP_F32)
-    hipLaunchKernelGGL(k_build_cquads<dbmp::kF32>, grid, block, 0, st, b->colors_orig,
-                       (long long)b->num_templates, b->height, b->width, dst);
+    hipLaunchKernelGGL(k_build_rgba<dbmp::kF32>, grid, block, 0, st, b->templates, b->colors_orig,
+                       b->num_templates, b->height, b->width, dst);
   else
-    hipLaunchKernelGGL(k_build_cquads<dbmp::kF16>, grid, block, 0, st, b->colors_orig,
-                       (long long)b->num_templates, b->height, b->width, dst);
-  LAUNCH_CHECK("k_build_cquads");
+    hipLaunchKernelGGL(k_build_rgba<dbmp::kF16>, grid, block, 0, st, b->templates, b->colors_orig,
+                       b->num_templates, b->height, b->width, dst);
+  LAUNCH_CHECK("k_build_rgba");
   return 0;
 }
 
-inline size_t cquad_bytes(int dtype) { return dtype == DBMP_F32 ? 64 : 32; }
+// Bytes per texel of the alpha quads / RGBA records.
+inline size_t quad_bytes(int dtype) { return dtype == DBMP_F32 ? 16 : 8; }
+inline size_t rgba_bytes(int dtype) { return dtype == DBMP_F32 ? 64 : 32; }
 
 }  // namespace dbmp_host

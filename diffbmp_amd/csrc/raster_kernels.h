@@ -36,9 +36,10 @@ struct RasterArgs {
   const long long* pcount;  // binning pair count (device) or null
   long long pcap;           // capacity the lists were built with
   const void* quads;
-  const void* cmap;      // colour maps or null; layout by cmap_mode
-  int cmap_mode;         // 0: (T,Ht,Wt,3) by template, 1: (N,Ht,Wt,3) by primitive,
-                         // 2: colour-map quads (T,Ht,Wt) of the per-template maps
+  const void* cmap;      // colour maps, or null: no colour-map blend
+  int cmap_mode;         // 1: cmap = (N,Ht,Wt,3) by primitive, quads = alpha quads;
+                         // 2: quads = RGBA texel records (RecT) of the per-template
+                         //    maps (cmap = quads when blending)
   float c_blend;
   int exact_blend;       // 1: gradient of the blended colour w.r.t. logits and geometry;
                          // 0: the reference's (no (1 - c_blend), no colour-map term)
@@ -69,9 +70,14 @@ struct CCorners {
   float c00[3], c10[3], c01[3], c11[3];   // (x0,y0), (x1,y0), (x0,y1), (x1,y1)
 };
 
+// Colour corners of a sample at texel coordinates (cx, cy): from the RGBA
+// record the sample's gather already brought in (cmap_mode 2; a pushed sample
+// has x0 = floor(tx), the record's texel), or gathered from per-primitive
+// maps (cmap_mode 1).
 template <int DT>
 __device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const PrimRec& p, int n,
-                                                 float cx, float cy) {
+                                                 float cx, float cy,
+                                                 const typename RecT<DT>::Col& raw) {
   CCorners k;
   cx = fmaxf(0.f, fminf(A.Wtm1, cx));
   cy = fmaxf(0.f, fminf(A.Htm1, cy));
@@ -79,12 +85,15 @@ __device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const Prim
   k.fx = cx - (float)x0;
   k.fy = cy - (float)y0;
   if (A.cmap_mode == 2) {
-    using Q = CQuad<DT>;
-    const long i = ((long)p.tmpl * A.Ht + y0) * A.Wt + x0;
-    Q::get((const typename Q::type*)A.cmap, i, k.c00, k.c10, k.c01, k.c11);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      k.c00[ch] = RecT<DT>::at(raw, ch);
+      k.c10[ch] = RecT<DT>::at(raw, 3 + ch);
+      k.c01[ch] = RecT<DT>::at(raw, 6 + ch);
+      k.c11[ch] = RecT<DT>::at(raw, 9 + ch);
+    }
   } else {
-    const long m = A.cmap_mode == 1 ? (long)n : (long)p.tmpl;
-    const StoreT<DT>* cm = (const StoreT<DT>*)A.cmap + m * (long)A.Ht * A.Wt * 3;
+    const StoreT<DT>* cm = (const StoreT<DT>*)A.cmap + (long)n * A.Ht * A.Wt * 3;
     const int x1 = min(x0 + 1, A.Wt - 1), y1 = min(y0 + 1, A.Ht - 1);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
@@ -100,8 +109,9 @@ __device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const Prim
 // Blended colour (1 - k) sig(c) + k c_o and, with GRAD, dc_o/d(tx, ty).
 template <int DT, bool GRAD>
 __device__ __forceinline__ void blend_color(const RasterArgs& A, const PrimRec& p, int n, float tx,
-                                            float ty, float col[3], float dcx[3], float dcy[3]) {
-  const CCorners k = cmap_corners<DT>(A, p, n, tx, ty);
+                                            float ty, const typename RecT<DT>::Col& raw,
+                                            float col[3], float dcx[3], float dcy[3]) {
+  const CCorners k = cmap_corners<DT>(A, p, n, tx, ty, raw);
   const float pc[3] = {p.cr, p.cg, p.cb};
   const float kb = A.c_blend, k1 = 1.f - A.c_blend;
 #pragma unroll
@@ -114,18 +124,6 @@ __device__ __forceinline__ void blend_color(const RasterArgs& A, const PrimRec& 
       dcx[ch] = fmaf(k.fy, (k.c11[ch] - k.c01[ch]) - (k.c10[ch] - k.c00[ch]), k.c10[ch] - k.c00[ch]);
       dcy[ch] = c1 - c0;
     }
-  }
-}
-
-template <int DT, bool BLEND>
-__device__ __forceinline__ void sample_color(const RasterArgs& A, const PrimRec& p, int n,
-                                             float tx, float ty, float col[3]) {
-  // a batch mixing canvases with and without colour maps runs BLEND kernels;
-  // a canvas without one keeps its plain colours
-  if (BLEND && A.cmap != nullptr) {
-    blend_color<DT, false>(A, p, n, tx, ty, col, nullptr, nullptr);
-  } else {
-    col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
   }
 }
 
@@ -222,9 +220,10 @@ template <int DT>
 struct Sample {
   float m, gxt, gyt, u, v, ndx, ndy, tx, ty;
   bool pushed;
+  typename RecT<DT>::Col craw;   // colour part of an RGBA record (BLEND, cmap_mode 2)
 };
 
-template <int DT, bool GRAD>
+template <int DT, bool GRAD, bool BLEND>
 __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const PrimRec& p, float fpx,
                                                   float fpy, bool active) {
   using Q = QuadT<DT>;
@@ -246,9 +245,15 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   // texel index within the template in float (exact below 2^24), clamped so
   // that NaN or out-of-range coordinates still address a texel of the template
   const float fl = __builtin_amdgcn_fmed3f(fmaf(fy, A.Wtm1 + 1.f, fx), 0.f, A.HWm1f);
-  const unsigned idx = (unsigned)(p.tbase + (int)fl);
+  // with colour maps (small, minified primitives are typical there) a lane
+  // that cannot push reads the template's first texel, so that idle lanes do
+  // not spread cache lines over the bitmap
+  const unsigned idx = (unsigned)(p.tbase + ((!BLEND || ok) ? (int)fl : 0));
   float v00, v01, v10, v11;
-  Q::get((const typename Q::type*)A.quads, idx, v00, v01, v10, v11);
+  if (BLEND && A.cmap_mode == 2)
+    RecT<DT>::get(A.quads, idx, v00, v01, v10, v11, r.craw);
+  else
+    Q::get((const typename Q::type*)A.quads, idx, v00, v01, v10, v11);
   const float wx = r.tx - fx, wy = r.ty - fy;
   float m;
   if constexpr (GRAD) {
@@ -298,13 +303,19 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         const int j = __builtin_ctzll(wm);
         wm &= wm - 1ull;
         const PrimRec p = srec[buf][j];
-        const Sample<DT> S = eval_sample<DT, false>(A, p, fpx, fpy, alive);
+        const Sample<DT> S = eval_sample<DT, false, BLEND>(A, p, fpx, fpy, alive);
         if constexpr (BLEND) {
           // skip the colour-map gathers when nothing of this entry reaches the wave
           if (!__any(S.pushed)) continue;
         }
         float col[3];
-        sample_color<DT, BLEND>(A, p, sid[buf][j], S.tx, S.ty, col);
+        if (BLEND && A.cmap != nullptr) {
+          blend_color<DT, false>(A, p, sid[buf][j], S.tx, S.ty, S.craw, col, nullptr, nullptr);
+        } else {
+          // a batch mixing canvases with and without colour maps runs BLEND
+          // kernels; a canvas without one keeps its plain colours
+          col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
+        }
         const float a = S.pushed ? p.abase * S.m : 0.f;   // a = 0: compositing is a no-op
         const float wgt = T * a;
         C0 = fmaf(wgt, col[0], C0);
@@ -438,12 +449,12 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       wm &= wm - 1ull;
       const int kk = base - start + j;
       const PrimRec p = srec[buf][j];
-      const Sample<DT> S = eval_sample<DT, true>(A, p, fpx, fpy, kk < lst);
+      const Sample<DT> S = eval_sample<DT, true, BLEND>(A, p, fpx, fpy, kk < lst);
       if (!__any(S.pushed)) continue;
       float col[3], dcx[3] = {0.f, 0.f, 0.f}, dcy[3] = {0.f, 0.f, 0.f};
       const bool has_map = BLEND && A.cmap != nullptr;
       if (has_map) {
-        blend_color<DT, true>(A, p, sid[buf][j], S.tx, S.ty, col, dcx, dcy);
+        blend_color<DT, true>(A, p, sid[buf][j], S.tx, S.ty, S.craw, col, dcx, dcy);
       } else {
         col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
       }

@@ -31,7 +31,7 @@ from . import _native
 from ._native import Bitmaps, Job, check
 
 _QUAD_BYTES = {torch.float32: 16, torch.float16: 8, torch.bfloat16: 8}
-_CQUAD_BYTES = {torch.float32: 64, torch.float16: 32, torch.bfloat16: 32}
+_RGBA_BYTES = {torch.float32: 64, torch.float16: 32, torch.bfloat16: 32}
 
 
 @dataclass
@@ -60,9 +60,8 @@ def job_bytes(spec: JobSpec, store_dtype=torch.float16) -> int:
     Ht = Wt = spec.template_size
     es = torch.tensor([], dtype=store_dtype).element_size()
     b = N * 8 * 4 * 3 + N * 8 * es + N * 4          # master, moments, store, sel
-    b += T * Ht * Wt * (es + 3 * es + _QUAD_BYTES[store_dtype]) + T * 16  # bitmaps, maps, quads, ext
-    if spec.c_blend > 0:
-        b += T * Ht * Wt * _CQUAD_BYTES[store_dtype]                      # colour-map quads
+    rec = _RGBA_BYTES[store_dtype] if spec.c_blend > 0 else _QUAD_BYTES[store_dtype]
+    b += T * Ht * Wt * (es + 3 * es + rec) + T * 16     # bitmaps, maps, texel records, extents
     b += H * W * (3 + 3 + 1 + 1 + 1) * 4             # target, colour, alpha, T, last
     b += spec.steps * 4                              # loss history
     return int(b)
@@ -153,16 +152,16 @@ class FitState:
         self.templates = torch.from_numpy(lib_t[ids]).to(dev).to(sdt).contiguous()
         self.colors_orig = torch.from_numpy(lib_c[ids]).to(dev).to(sdt).contiguous()
         self.ext = torch.empty((T, 4), dtype=torch.float32, device=dev)
-        self.quads = torch.empty(T * Ht * Wt * _QUAD_BYTES[sdt], dtype=torch.uint8, device=dev)
-        blend = spec.c_blend > 0
-        self.cquads = torch.empty(T * Ht * Wt * _CQUAD_BYTES[sdt], dtype=torch.uint8,
-                                  device=dev) if blend else None
-        bm = Bitmaps(self.templates.data_ptr(), self.colors_orig.data_ptr() if blend else None,
-                     T, Ht, Wt, _native._DT[sdt], 2 if blend else 0, float(spec.c_blend), -1, 0, -1)
+        self.blend = spec.c_blend > 0
+        rec = _RGBA_BYTES[sdt] if self.blend else _QUAD_BYTES[sdt]
+        self.quads = torch.empty(T * Ht * Wt * rec, dtype=torch.uint8, device=dev)
+        bm = Bitmaps(self.templates.data_ptr(), self.colors_orig.data_ptr() if self.blend else None,
+                     T, Ht, Wt, _native._DT[sdt], 2 if self.blend else 0, float(spec.c_blend), -1, 0,
+                     -1)
         with torch.cuda.device(dev):
             check(batch._L.dbmp_batch_prepare_bitmaps(batch._b, ctypes.byref(bm),
                                                      _native._ptr(self.ext), _native._ptr(self.quads),
-                                                     _native._ptr(self.cquads), _native._stream(dev)))
+                                                     _native._stream(dev)))
         lo, hi = spec.ls_range
         x = rng.uniform(0, W, N)
         y = rng.uniform(0, H, N)
@@ -197,7 +196,7 @@ class FitState:
         j.height, j.width, j.num_primitives = s.height, s.width, s.num_primitives
         j.num_templates, j.tmpl_height, j.tmpl_width = T, Ht, Wt
         j.quads, j.ext = self.quads.data_ptr(), self.ext.data_ptr()
-        j.color_quads = self.cquads.data_ptr() if self.cquads is not None else None
+        j.quads_with_color = int(self.blend)
         j.c_blend = float(s.c_blend)
         j.blend_grad_exact = int(exact_blend)
         j.bmp_sel = self.sel.data_ptr()

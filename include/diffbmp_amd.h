@@ -226,7 +226,7 @@ typedef struct {
   int32_t num_templates, tmpl_height, tmpl_width;
   const void* quads;            /* prepared by dbmp_batch_prepare_bitmaps */
   const float* ext;             /* (T,4) template extents, likewise */
-  const void* color_quads;      /* colour-map quads of the per-template maps, likewise, or NULL */
+  int32_t quads_with_color;     /* 1: `quads` were prepared with per-template colour maps */
   float c_blend;
   int32_t blend_grad_exact;     /* as in dbmp_bitmaps */
   const int32_t* bmp_sel;       /* (N) */
@@ -255,12 +255,13 @@ typedef struct {
 int dbmp_batch_create(dbmp_batch** out, int32_t storage_dtype, float beta1, float beta2, float eps);
 int dbmp_batch_destroy(dbmp_batch* b);
 
-/* Template extents (T,4), texel quads (T*Ht*Wt entries of 16 bytes for fp32
- * storage, 8 bytes otherwise) and, when bm carries per-template colour maps
- * (colors_orig_mode 2), colour-map quads (T*Ht*Wt entries of 64 / 32 bytes)
- * of one template set, for dbmp_job. */
+/* Template extents (T,4) and texel records of one template set, for
+ * dbmp_job: T*Ht*Wt alpha quads of 16 bytes (fp32 storage) or 8 bytes, or,
+ * when bm carries per-template colour maps (colors_orig_mode 2), T*Ht*Wt RGBA
+ * records of 64 / 32 bytes holding each texel's alpha and colour
+ * neighbourhoods (set quads_with_color in the job). */
 int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext_out,
-                               void* quads_out, void* color_quads_out, void* stream);
+                               void* quads_out, void* stream);
 
 /* Replace the job set (copies the descriptors; jobs keep their own device
  * buffers).  pair_capacity bounds the (tile, primitive) pairs of all jobs
