@@ -56,7 +56,8 @@ class Bitmaps(ctypes.Structure):
 
 class TileLists(ctypes.Structure):
     _fields_ = [("tile_offsets", ctypes.c_void_p), ("tile_indices", ctypes.c_void_p),
-                ("pair_count", ctypes.c_void_p), ("capacity", ctypes.c_int64)]
+                ("pair_count", ctypes.c_void_p), ("capacity", ctypes.c_int64),
+                ("bin_factor", ctypes.c_int32), ("ranges", ctypes.c_void_p)]
 
 
 class Job(ctypes.Structure):
@@ -80,18 +81,22 @@ class Job(ctypes.Structure):
 
 class Lists:
     """Tile lists on the device: CSR offsets, index buffer, and -- for lists
-    produced by dbmp_bin -- the device pair count and the capacity used."""
+    produced by dbmp_bin -- the device pair count and the capacity used.
+    ``factor`` > 1 marks super-tile lists from dbmp_bin_tiles (one list per
+    factor x factor tiles) with the binning's per-primitive tile ``ranges``."""
 
-    __slots__ = ("offsets", "indices", "count", "capacity")
+    __slots__ = ("offsets", "indices", "count", "capacity", "factor", "ranges")
 
-    def __init__(self, offsets, indices, count=None, capacity=0):
+    def __init__(self, offsets, indices, count=None, capacity=0, factor=1, ranges=None):
         self.offsets, self.indices, self.count, self.capacity = offsets, indices, count, int(capacity)
+        self.factor, self.ranges = int(factor), ranges
 
     def c(self) -> TileLists:
         idx = self.indices if self.indices.numel() else self.offsets
         return TileLists(self.offsets.data_ptr(), idx.data_ptr(),
                          self.count.data_ptr() if self.count is not None else None,
-                         self.capacity)
+                         self.capacity, self.factor,
+                         self.ranges.data_ptr() if self.ranges is not None else None)
 
 
 EXPORTS = {
@@ -107,6 +112,10 @@ EXPORTS = {
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                 ctypes.c_void_p]),
+    "dbmp_bin_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims), ctypes.c_void_p,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int32,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dbmp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims),
                                     ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
                                     ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
@@ -214,6 +223,9 @@ class Handle:
         self._h = h
         self._L = L
         self.pair_capacity = 0
+        # internal binning granularity: 64-px super-tiles for 16-px tiles
+        # (same per-tile work, a sixteenth of the pairs to sort)
+        self.bin_factor = 4 if self.ts == 16 else 1
         self._pending = None          # (event, pinned count) of the last async binning
         self._tmpl_cache = None       # (key, converted templates, extents, version)
         self._tmpl_gen = 0
@@ -272,30 +284,52 @@ class Handle:
                                             _stream(templates.device)))
         return ext
 
+    def list_count(self, factor: int = 1) -> int:
+        """Number of tile lists binning with ``factor`` produces."""
+        return (-(-self.tilesX // factor)) * (-(-self.tilesY // factor))
+
     def bin_async(self, prims: Prims, ext: torch.Tensor, tmpl_shape, capacity: int,
                   tile_offsets: torch.Tensor, tile_indices: torch.Tensor,
-                  pair_count: torch.Tensor) -> None:
-        """Launch binning into caller buffers; no host synchronisation."""
+                  pair_count: torch.Tensor, factor: int = 1,
+                  ranges: torch.Tensor | None = None) -> None:
+        """Launch binning into caller buffers; no host synchronisation.
+        ``factor`` > 1 bins per super-tile of factor x factor tiles and writes
+        each primitive's tile range into ``ranges`` ((N,4) int32)."""
         T, Ht, Wt = tmpl_shape
+        if factor > 1:
+            check(self._L.dbmp_bin_tiles(self._h, ctypes.byref(prims), _ptr(ext), int(T), int(Ht),
+                                         int(Wt), int(factor), _ptr(tile_offsets),
+                                         _ptr(tile_indices), int(capacity), _ptr(pair_count),
+                                         _ptr(ranges), _stream(ext.device)))
+            return
         check(self._L.dbmp_bin(self._h, ctypes.byref(prims), _ptr(ext), int(T), int(Ht), int(Wt),
                                _ptr(tile_offsets), _ptr(tile_indices), int(capacity),
                                _ptr(pair_count), _stream(ext.device)))
 
-    def bin(self, prims: Prims, ext: torch.Tensor, tmpl_shape):
-        """Tile lists (offsets (tiles+1) int32, indices (P) int32).  Reads the
-        pair count back once (the only host synchronisation of the step) and
-        regrows the index buffer if it overflowed."""
+    def _ranges(self, prims: Prims, dev, factor: int):
+        if factor <= 1:
+            return None
+        return torch.empty((max(prims.num_primitives, 1), 4), dtype=torch.int32, device=dev)
+
+    def bin(self, prims: Prims, ext: torch.Tensor, tmpl_shape, factor: int = 1):
+        """Tile lists (offsets (lists+1) int32, indices (P) int32; with
+        ``factor`` > 1 also the (N,4) tile ranges).  Reads the pair count back
+        once (the only host synchronisation of the step) and regrows the index
+        buffer if it overflowed."""
         dev = ext.device
         N = prims.num_primitives
         cap = max(self.pair_capacity, 4 * N + 1024)
-        offs = torch.empty(self.tiles + 1, dtype=torch.int32, device=dev)
+        offs = torch.empty(self.list_count(factor) + 1, dtype=torch.int32, device=dev)
         cnt = torch.empty(1, dtype=torch.int64, device=dev)
+        rg = self._ranges(prims, dev, factor)
         for _ in range(2):
             idx = torch.empty(cap, dtype=torch.int32, device=dev)
-            self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt)
+            self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
             P = int(cnt.item())
             if P <= cap:
                 self.pair_capacity = max(self.pair_capacity, cap)
+                if factor > 1:
+                    return offs, idx[:P], rg
                 return offs, idx[:P]
             cap = int(math.ceil(P * 1.25)) + 1024
             if cap >= 2 ** 31:
@@ -319,7 +353,7 @@ class Handle:
         self._tmpl_cache = (key, conv, ext, self._tmpl_gen)
         return conv, ext, self._tmpl_gen
 
-    def bin_lazy(self, prims: Prims, ext: torch.Tensor, tmpl_shape) -> Lists:
+    def bin_lazy(self, prims: Prims, ext: torch.Tensor, tmpl_shape, factor: int = 1) -> Lists:
         """Tile lists without a host synchronisation in steady state.  The
         capacity follows the pair counts seen so far (read back
         asynchronously); the first call, with no history, reads the count
@@ -333,21 +367,22 @@ class Handle:
                 self.pair_capacity = int(seen * 1.3) + 4096
             self._pending = None
         if self.pair_capacity == 0:
-            offs, idx = self.bin(prims, ext, tmpl_shape)
-            self.pair_capacity = int(idx.numel() * 1.3) + 4096
-            return Lists(offs, idx)
+            out = self.bin(prims, ext, tmpl_shape, factor)
+            self.pair_capacity = int(out[1].numel() * 1.3) + 4096
+            return Lists(out[0], out[1], factor=factor, ranges=out[2] if factor > 1 else None)
         cap = self.pair_capacity
-        offs = torch.empty(self.tiles + 1, dtype=torch.int32, device=dev)
+        offs = torch.empty(self.list_count(factor) + 1, dtype=torch.int32, device=dev)
         idx = torch.empty(cap, dtype=torch.int32, device=dev)
         cnt = torch.empty(1, dtype=torch.int64, device=dev)
-        self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt)
+        rg = self._ranges(prims, dev, factor)
+        self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
         if self._pending is None:
             host = torch.empty(1, dtype=torch.int64, pin_memory=True)
             host.copy_(cnt, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(torch.cuda.current_stream(dev))
             self._pending = (ev, host)
-        return Lists(offs, idx, cnt, cap)
+        return Lists(offs, idx, cnt, cap, factor, rg)
 
     def forward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T: float, out_color, out_alpha,
                 out_T, out_last) -> None:

@@ -357,8 +357,6 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.Ht = s.tmpl_height;
     A.Wt = s.tmpl_width;
     A.HWm1f = (float)(s.tmpl_height * s.tmpl_width - 1);
-    A.Wtm1 = (float)(s.tmpl_width - 1);
-    A.Htm1 = (float)(s.tmpl_height - 1);
     A.hsx = 0.5f * (float)(s.tmpl_width - 1);
     A.hsy = 0.5f * (float)(s.tmpl_height - 1);
     A.H = s.height;
@@ -368,7 +366,9 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.tiles = ts[i + 1] - ts[i];
     A.subs = 1;
     A.eps = s.eps_T;
-    A.alpha_ub = s.alpha_upper_bound;
+    A.ranges = nullptr;
+    A.bin_shift = 0;
+    d.alpha_ub = s.alpha_upper_bound;
     d.L.target = s.target;
     d.L.bg = s.background;
     d.L.bg_value = s.background_value;

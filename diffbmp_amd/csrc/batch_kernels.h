@@ -43,6 +43,7 @@ struct BatchJob {
   int* last;           // (H,W) entries consumed | opaque count << 30
   float* loss_hist;    // (loss_cap) MSE per step
   int loss_cap;
+  float alpha_ub;      // alpha upper bound of the job
   float lr[5];         // means2D, radii, rotations, opacities, colors
   float lo[3], hi[3];  // clamp of x, y, r (lo > hi: no clamp)
 };
@@ -92,7 +93,7 @@ static __global__ __launch_bounds__(256) void k_batch_prep(const BatchJob* __res
   int4 rg;
   const RasterArgs& A = B.A;
   prep_core(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], B.sel[n], B.ext, B.T, A.Ht, A.Wt, A.H,
-            A.W, A.ts, A.tilesX, B.tilesY, A.alpha_ub, p, cnt, rg);
+            A.W, A.ts, A.tilesX, B.tilesY, B.alpha_ub, 0, p, cnt, rg);
   recs[g] = p;
   counts[g] = cnt;
   ranges[g] = rg;

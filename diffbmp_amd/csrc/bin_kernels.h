@@ -92,12 +92,13 @@ __device__ __forceinline__ void tile_interval(float bmin, float bmax, int nt, in
 }
 
 // Raster record and tile range of one primitive (parameters already in fp32).
-// ext == nullptr: raster constants only (cnt = 0).
+// ext == nullptr: raster constants only (cnt = 0).  rg is the range of
+// ts-px tiles; cnt counts the super-tiles of ts << bin_shift px it covers.
 __device__ __forceinline__ void prep_core(float x, float y, float r, float th, float v, float c0,
                                           float c1, float c2, int t, const float* __restrict__ ext,
                                           int T, int Ht, int Wt, int H, int W, int ts, int tilesX,
-                                          int tilesY, float alpha_ub, PrimRec& p, int& cnt,
-                                          int4& rg) {
+                                          int tilesY, float alpha_ub, int bin_shift, PrimRec& p,
+                                          int& cnt, int4& rg) {
   // correctly rounded fp32 trig, shared by binning and raster
   const float c = (float)cos((double)th), s = (float)sin((double)th);
   p.mx = x; p.my = y; p.r = r; p.c = c; p.s = s;
@@ -145,7 +146,7 @@ __device__ __forceinline__ void prep_core(float x, float y, float r, float th, f
     tile_interval(xmn, xmx, tilesX, ts, W, tx0, tx1);
     tile_interval(ymn, ymx, tilesY, ts, H, ty0, ty1);
     if (tx1 >= tx0 && ty1 >= ty0) {
-      cnt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+      cnt = ((tx1 >> bin_shift) - (tx0 >> bin_shift) + 1) * ((ty1 >> bin_shift) - (ty0 >> bin_shift) + 1);
       rg = make_int4(tx0, tx1, ty0, ty1);
     }
   }
@@ -157,7 +158,8 @@ static __global__ __launch_bounds__(256) void k_prep(
     const PT* __restrict__ rotations, const PT* __restrict__ opacities,
     const PT* __restrict__ colors, const int* __restrict__ sel, const float* __restrict__ ext,
     int T, int Ht, int Wt, int H, int W, int ts, int tilesX, int tilesY, float alpha_ub,
-    PrimRec* __restrict__ recs, int* __restrict__ counts, int4* __restrict__ ranges) {
+    int bin_shift, PrimRec* __restrict__ recs, int* __restrict__ counts,
+    int4* __restrict__ ranges) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   PrimRec p;
@@ -166,10 +168,12 @@ static __global__ __launch_bounds__(256) void k_prep(
   prep_core(ldf(means2D, 2L * n), ldf(means2D, 2L * n + 1), ldf(radii, n), ldf(rotations, n),
             ldf(opacities, n), ldf(colors, 3L * n), ldf(colors, 3L * n + 1),
             ldf(colors, 3L * n + 2), sel[n], ext, T, Ht, Wt, H, W, ts, tilesX, tilesY, alpha_ub,
-            p, cnt, rg);
+            bin_shift, p, cnt, rg);
   recs[n] = p;
-  counts[n] = cnt;
-  ranges[n] = rg;
+  if (ext != nullptr) {   // the binning's ranges stay valid across raster-only calls
+    counts[n] = cnt;
+    ranges[n] = rg;
+  }
 }
 
 // Inclusive wave scan + block exclusive scan over 256 threads.
@@ -261,9 +265,10 @@ static __global__ __launch_bounds__(kScanThreads) void k_scan_counts(const int* 
 }
 
 // One wave per primitive; pairs land at offs[n] in row-major tile order.
+// tilesX: tiles per row of the list grid (super-tiles of 2^shift tiles).
 static __global__ __launch_bounds__(256) void k_emit(int N, const int* __restrict__ counts,
                                               const long long* __restrict__ offs,
-                                              const int4* __restrict__ ranges, int tilesX,
+                                              const int4* __restrict__ ranges, int tilesX, int shift,
                                               const long long* __restrict__ total, long long cap,
                                               int* __restrict__ keys, int* __restrict__ vals) {
   if (*total > cap) return;
@@ -272,7 +277,8 @@ static __global__ __launch_bounds__(256) void k_emit(int N, const int* __restric
   const int cnt = counts[n];
   if (cnt == 0) return;
   const long long off = offs[n];
-  const int4 rg = ranges[n];
+  const int4 r16 = ranges[n];
+  const int4 rg = make_int4(r16.x >> shift, r16.y >> shift, r16.z >> shift, r16.w >> shift);
   const int nx = rg.y - rg.x + 1;
   for (int i = lane_id(); i < cnt; i += 64) {
     const int qy = i / nx, qx = i - qy * nx;

@@ -136,8 +136,10 @@ int check_bitmaps(const dbmp_bitmaps* b) {
   return 0;
 }
 
+// ranges: where the binning's tile ranges go (h->ranges when null); shift:
+// log2 of the super-tile factor the pair counts are for.
 int launch_prep(dbmp_rasterizer* h, const dbmp_prims* p, const float* ext, int T, int Ht, int Wt,
-                hipStream_t st) {
+                hipStream_t st, int4* ranges = nullptr, int shift = 0) {
   const int N = p->num_primitives;
   if (ext == nullptr && p->reuse_records && h->recs_n == N) return 0;
   h->recs_n = N;
@@ -146,7 +148,8 @@ int launch_prep(dbmp_rasterizer* h, const dbmp_prims* p, const float* ext, int T
 #define PREP_ARGS(TY)                                                                          \
   N, (const TY*)p->means2D, (const TY*)p->radii, (const TY*)p->rotations,                      \
       (const TY*)p->opacities, (const TY*)p->colors, p->bmp_sel, ext, T, Ht, Wt, h->H, h->W,   \
-      h->ts, h->tilesX, h->tilesY, h->alpha_ub, h->recs, h->counts, h->ranges
+      h->ts, h->tilesX, h->tilesY, h->alpha_ub, shift, h->recs, h->counts,                    \
+      ranges ? ranges : h->ranges
   if (p->dtype == DBMP_F32)
     hipLaunchKernelGGL(k_prep<float>, grid, block, 0, st, PREP_ARGS(float));
   else if (p->dtype == DBMP_F16)
@@ -211,14 +214,14 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   // per-template maps are sampled from the RGBA records in `quads`
   A.cmap = !blend ? nullptr : (b->colors_orig_mode == 2 ? h->quads : b->colors_orig);
   A.cmap_mode = (blend && b->colors_orig_mode == 2) ? 2 : 1;
+  A.ranges = nullptr;
+  A.bin_shift = 0;
   A.c_blend = b->c_blend;
   A.exact_blend = b->blend_grad_exact != 0;
   A.N = p->num_primitives;
   A.Ht = b->height;
   A.Wt = b->width;
   A.HWm1f = (float)(b->height * b->width - 1);
-  A.Wtm1 = (float)(b->width - 1);
-  A.Htm1 = (float)(b->height - 1);
   A.hsx = 0.5f * (float)(b->width - 1);
   A.hsy = 0.5f * (float)(b->height - 1);
   A.H = h->H;
@@ -228,8 +231,31 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.tiles = h->tiles;
   A.subs = h->subs;
   A.eps = eps;
-  A.alpha_ub = h->alpha_ub;
+  const int f = L->bin_factor;
+  if (f > 1) {   // super-tile lists (checked by check_lists)
+    int s = 0;
+    while ((1 << s) < f) ++s;
+    A.ranges = (const int4*)L->ranges;
+    A.bin_shift = s;
+    A.ts = h->ts * f;
+    A.tilesX = (h->tilesX + f - 1) / f;
+    A.tiles = A.tilesX * ((h->tilesY + f - 1) / f);
+    A.subs = f * f;
+  }
   return A;
+}
+
+int check_lists(dbmp_rasterizer* h, const dbmp_tile_lists* L) {
+  if (!L || !L->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  const int f = L->bin_factor;
+  if (f > 1) {
+    if (f != 2 && f != 4 && f != 8) return fail(DBMP_E_ARG, "bin_factor must be 1, 2, 4 or 8");
+    if (h->ts != kSub) return fail(DBMP_E_ARG, "bin_factor > 1 needs tile size 16");
+    if (!L->ranges) return fail(DBMP_E_ARG, "bin_factor > 1 needs the binning's ranges");
+  } else if (f < 0) {
+    return fail(DBMP_E_ARG, "bad bin_factor %d", f);
+  }
+  return 0;
 }
 
 void timing_begin(dbmp_rasterizer* h, std::vector<std::pair<hipEvent_t, hipEvent_t>>& v,
@@ -337,9 +363,10 @@ int dbmp_template_extents(dbmp_rasterizer* h, const dbmp_bitmaps* bm, float* ext
   return 0;
 }
 
-int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
-             int tmpl_h, int tmpl_w, int32_t* tile_offsets, int32_t* tile_indices,
-             int64_t capacity, int64_t* d_pair_count, void* stream) {
+int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
+                   int tmpl_h, int tmpl_w, int32_t bin_factor, int32_t* tile_offsets,
+                   int32_t* tile_indices, int64_t capacity, int64_t* d_pair_count,
+                   int32_t* ranges_out, void* stream) {
   if (!h) return fail(DBMP_E_ARG, "handle is NULL");
   int rc = check_prims(prims);
   if (rc) return rc;
@@ -348,17 +375,25 @@ int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int 
   if (capacity < 0 || capacity > (int64_t)0x7fffffff)
     return fail(DBMP_E_ARG, "capacity must be in [0, 2^31), got %lld", (long long)capacity);
   if (capacity > 0 && !tile_indices) return fail(DBMP_E_ARG, "tile_indices is NULL");
+  const int f = bin_factor < 1 ? 1 : bin_factor;
+  if (f != 1 && f != 2 && f != 4 && f != 8) return fail(DBMP_E_ARG, "bin_factor must be 1, 2, 4 or 8");
+  if (f > 1 && h->ts != kSub) return fail(DBMP_E_ARG, "bin_factor > 1 needs tile size 16");
+  if (f > 1 && !ranges_out) return fail(DBMP_E_ARG, "bin_factor > 1 needs ranges_out");
+  int shift = 0;
+  while ((1 << shift) < f) ++shift;
+  const int tX = (h->tilesX + f - 1) / f, tY = (h->tilesY + f - 1) / f, tiles = tX * tY;
   hipStream_t st = (hipStream_t)stream;
   const int N = prims->num_primitives;
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
   if ((rc = ensure_pairs(h, std::max<long long>(capacity, 1)))) return rc;
   long long* dtot = (long long*)d_pair_count;
-  if ((rc = launch_prep(h, prims, ext, num_templates, tmpl_h, tmpl_w, st))) return rc;
+  int4* rg = ranges_out ? (int4*)ranges_out : h->ranges;
+  if ((rc = launch_prep(h, prims, ext, num_templates, tmpl_h, tmpl_w, st, rg, shift))) return rc;
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, h->counts, N, h->offs, dtot);
   LAUNCH_CHECK("k_scan_counts");
 
   int bits = 0;
-  while ((1 << bits) < h->tiles) ++bits;
+  while ((1 << bits) < tiles) ++bits;
   const int passes = (bits + 7) / 8;
   const int dbits = passes ? (bits + passes - 1) / passes : 0;
   const bool even = (passes % 2) == 0;
@@ -367,29 +402,36 @@ int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int 
   int* kalt = h->keysB;
   int* valt = even ? h->valsB : tile_indices;
   if (N > 0 && capacity > 0) {
-    hipLaunchKernelGGL(k_emit, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, h->counts, h->offs,
-                       h->ranges, h->tilesX, dtot, (long long)capacity, kcur, vcur);
+    hipLaunchKernelGGL(k_emit, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, h->counts, h->offs, rg, tX,
+                       shift, dtot, (long long)capacity, kcur, vcur);
     LAUNCH_CHECK("k_emit");
     const int nblk = ceil_div(capacity, kRadixTile);
     for (int p = 0; p < passes; ++p) {
-      const int shift = p * dbits;
-      const int nb = std::min(dbits, bits - shift);
+      const int sh = p * dbits;
+      const int nb = std::min(dbits, bits - sh);
       hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, dtot,
-                         (long long)capacity, shift, nb, nblk, h->hist);
+                         (long long)capacity, sh, nb, nblk, h->hist);
       LAUNCH_CHECK("k_radix_hist");
       hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << nb), dim3(256), 0, st, h->hist, nblk, h->totals);
       LAUNCH_CHECK("k_radix_rowscan");
       hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, vcur, kalt,
-                         valt, dtot, (long long)capacity, shift, nb, nblk, h->hist, h->totals);
+                         valt, dtot, (long long)capacity, sh, nb, nblk, h->hist, h->totals);
       LAUNCH_CHECK("k_radix_scatter");
       std::swap(kcur, kalt);
       std::swap(vcur, valt);
     }
   }
-  hipLaunchKernelGGL(k_tile_bounds, dim3(ceil_div(std::max<long long>(capacity, h->tiles) + 1, 256)),
-                     dim3(256), 0, st, kcur, dtot, (long long)capacity, h->tiles, tile_offsets);
+  hipLaunchKernelGGL(k_tile_bounds, dim3(ceil_div(std::max<long long>(capacity, tiles) + 1, 256)),
+                     dim3(256), 0, st, kcur, dtot, (long long)capacity, tiles, tile_offsets);
   LAUNCH_CHECK("k_tile_bounds");
   return 0;
+}
+
+int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
+             int tmpl_h, int tmpl_w, int32_t* tile_offsets, int32_t* tile_indices,
+             int64_t capacity, int64_t* d_pair_count, void* stream) {
+  return dbmp_bin_tiles(h, prims, ext, num_templates, tmpl_h, tmpl_w, 1, tile_offsets, tile_indices,
+                        capacity, d_pair_count, nullptr, stream);
 }
 
 #define DISPATCH_DT_BLEND(DT, BLEND, KERNEL, ...)                                            \
@@ -413,7 +455,7 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   int rc = check_prims(prims);
   if (rc) return rc;
   if ((rc = check_bitmaps(bm))) return rc;
-  if (!lists || !lists->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  if ((rc = check_lists(h, lists))) return rc;
   if (!out_color || !out_alpha || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL output");
   if (!(eps_T >= 0.f)) return fail(DBMP_E_ARG, "eps_T must be >= 0");
   hipStream_t st = (hipStream_t)stream;
@@ -425,7 +467,7 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
   const RasterArgs A = make_args(h, prims, bm, lists, eps_T);
   const bool blend = A.cmap != nullptr;
-  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(h->tiles * h->subs), dim3(kThreads), 0, st, A,
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(A.tiles * A.subs), dim3(kThreads), 0, st, A,
                     out_color, out_alpha, out_T_final, out_last);
   LAUNCH_CHECK("k_raster_fwd");
   timing_end(h, h->ev_fwd, st);
@@ -442,7 +484,7 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   int rc = check_prims(prims);
   if (rc) return rc;
   if ((rc = check_bitmaps(bm))) return rc;
-  if (!lists || !lists->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  if ((rc = check_lists(h, lists))) return rc;
   if (!grad_color || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
   if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
     return fail(DBMP_E_ARG, "NULL gradient output");
@@ -458,7 +500,7 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   }
   const RasterArgs A = make_args(h, prims, bm, lists, 0.f);
   const bool blend = A.cmap != nullptr;
-  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd, dim3(h->tiles * h->subs), dim3(kThreads), 0, st, A,
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd, dim3(A.tiles * A.subs), dim3(kThreads), 0, st, A,
                     grad_color, grad_alpha, out_color, out_T_final, out_last, h->grad8);
   LAUNCH_CHECK("k_raster_bwd");
   if (N > 0) {
@@ -481,7 +523,7 @@ int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bi
   int rc = check_prims(prims);
   if (rc) return rc;
   if ((rc = check_bitmaps(bm))) return rc;
-  if (!lists || !lists->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  if ((rc = check_lists(h, lists))) return rc;
   if (!target || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
   if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
     return fail(DBMP_E_ARG, "NULL gradient output");
@@ -503,7 +545,7 @@ int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bi
   L.inv_count = 1.f / (float)(3.0 * h->H * h->W);
   L.loss = loss_accum;
   const bool blend = A.cmap != nullptr;
-  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd_mse, dim3(h->tiles * h->subs), dim3(kThreads), 0, st,
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd_mse, dim3(A.tiles * A.subs), dim3(kThreads), 0, st,
                     A, L, out_color, out_T_final, out_last, h->grad8);
   LAUNCH_CHECK("k_raster_bwd_mse");
   if (N > 0) {

@@ -37,6 +37,9 @@ struct RasterArgs {
   long long pcap;           // capacity the lists were built with
   const void* quads;
   const void* cmap;      // colour maps, or null: no colour-map blend
+  const int4* ranges;    // lists per super-tile of 16 << bin_shift px: each primitive's
+                         // 16-px tile range (tx0, tx1, ty0, ty1); an entry is used in
+                         // a 16x16 block only inside that range.  Null: per-tile lists
   int cmap_mode;         // 1: cmap = (N,Ht,Wt,3) by primitive, quads = alpha quads;
                          // 2: quads = RGBA texel records (RecT) of the per-template
                          //    maps (cmap = quads when blending)
@@ -45,10 +48,10 @@ struct RasterArgs {
                          // 0: the reference's (no (1 - c_blend), no colour-map term)
   int N, Ht, Wt;
   float HWm1f;           // Ht*Wt - 1: last texel of one template, as a float
-  float Wtm1, Htm1, hsx, hsy;
-  int H, W, ts, tilesX, tiles, subs;   // subs = (ts/16)^2 sub-tiles per tile
+  float hsx, hsy;        // (Wt-1)/2, (Ht-1)/2
+  int H, W, ts, tilesX, tiles, subs;   // ts: list tile; subs = (ts/16)^2 blocks per tile
   float eps;
-  float alpha_ub;
+  int bin_shift;         // log2(list tile / 16) when ranges is set
 };
 
 // blockIdx -> (tile, sub-tile).  Consecutive workgroups are dealt round-robin
@@ -79,8 +82,8 @@ __device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const Prim
                                                  float cx, float cy,
                                                  const typename RecT<DT>::Col& raw) {
   CCorners k;
-  cx = fmaxf(0.f, fminf(A.Wtm1, cx));
-  cy = fmaxf(0.f, fminf(A.Htm1, cy));
+  cx = fmaxf(0.f, fminf((float)(A.Wt - 1), cx));
+  cy = fmaxf(0.f, fminf((float)(A.Ht - 1), cy));
   const int x0 = (int)cx, y0 = (int)cy;
   k.fx = cx - (float)x0;
   k.fy = cy - (float)y0;
@@ -196,10 +199,13 @@ __device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
 struct Stage {
   float4 v;
   int n;
+  int4 rg;   // 16-px tile range (super-tile lists)
 };
 __device__ __forceinline__ Stage stage_load(const RasterArgs& A, bool all, int k, int end, int part) {
   Stage st;
   st.v = fetch_rec_part(A, all, k, end, part, st.n);
+  st.rg = make_int4(0, 0x7fffffff, 0, 0x7fffffff);
+  if (A.ranges != nullptr && part == 0 && st.n >= 0) st.rg = A.ranges[st.n];
   return st;
 }
 __device__ __forceinline__ void stage_store(const Stage& st, PrimRec (*srec)[kBatch], int (*sid)[kBatch],
@@ -208,7 +214,11 @@ __device__ __forceinline__ void stage_store(const Stage& st, PrimRec (*srec)[kBa
   reinterpret_cast<float4*>(&srec[buf][ri])[part] = st.v;
   if (part == 0) {
     sid[buf][ri] = st.n;
-    smask[buf][ri] = wave_mask(st.v, bx, by);
+    // the entry belongs to this block's 16-px tile's list only inside the
+    // primitive's binning range (the reference's per-tile lists)
+    const int tx = bx >> 4, ty = by >> 4;
+    const bool in = (tx >= st.rg.x) & (tx <= st.rg.y) & (ty >= st.rg.z) & (ty <= st.rg.w);
+    smask[buf][ri] = in ? wave_mask(st.v, bx, by) : 0;
   }
 }
 
@@ -237,14 +247,15 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   r.v = -p.s * r.ndx + p.c * r.ndy;
   r.tx = (r.u + 1.f) * A.hsx;
   r.ty = (r.v + 1.f) * A.hsy;
-  const bool inb = (r.tx >= 0.f) & (r.tx <= A.Wtm1) & (r.ty >= 0.f) & (r.ty <= A.Htm1);
+  const bool inb = (r.tx >= 0.f) & (r.tx <= (float)(A.Wt - 1)) & (r.ty >= 0.f) &
+                   (r.ty <= (float)(A.Ht - 1));
   const bool ok = active & inbox & inb;
   const float fx = floorf(r.tx), fy = floorf(r.ty);
   // clamped rather than selected, so the gather is unconditional; a lane
   // that is not `ok` reads some valid quad and its sample is discarded
   // texel index within the template in float (exact below 2^24), clamped so
   // that NaN or out-of-range coordinates still address a texel of the template
-  const float fl = __builtin_amdgcn_fmed3f(fmaf(fy, A.Wtm1 + 1.f, fx), 0.f, A.HWm1f);
+  const float fl = __builtin_amdgcn_fmed3f(fmaf(fy, (float)A.Wt, fx), 0.f, A.HWm1f);
   // with colour maps (small, minified primitives are typical there) a lane
   // that cannot push reads the template's first texel, so that idle lanes do
   // not spread cache lines over the bitmap
@@ -278,6 +289,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   __shared__ int sid[2][kBatch];
   __shared__ int smask[2][kBatch];
   const BlockGeom G = block_geom(A, b);
+  if (G.bx >= A.W || G.by >= A.H) return;   // 16-px block of a tile past the canvas edge
   const float fpx = (float)G.px, fpy = (float)G.py;
   bool alive = (G.px < A.W) && (G.py < A.H);
   // T: transmittance; Tnz: product of the nonzero (1 - a) factors; nz: count
@@ -387,6 +399,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   __shared__ float sred[4][kBatch][8];
   __shared__ int skmax;
   const BlockGeom G = block_geom(A, b);
+  if (G.bx >= A.W || G.by >= A.H) return;   // 16-px block of a tile past the canvas edge
   const float fpx = (float)G.px, fpy = (float)G.py;
   const bool okpix = (G.px < A.W) && (G.py < A.H);
   const long pix = okpix ? (long)G.py * A.W + G.px : 0;

@@ -113,7 +113,7 @@ class FitJob:
             grown[: self.losses.numel()] = self.losses
             self.losses = grown
         h = self.handle
-        lists = h.bin_lazy(self._prims(False), self.ext, tuple(self.templates.shape))
+        lists = h.bin_lazy(self._prims(False), self.ext, tuple(self.templates.shape), h.bin_factor)
         self.lists = lists
         h.forward(self._prims(True), self._bm, lists, self.eps_T, self.color, self.alpha, self.T, self.last)
         g = self.grads
@@ -128,7 +128,7 @@ class FitJob:
     def render(self) -> torch.Tensor:
         """Composite image (H,W,3) of the current parameters over the background."""
         h = self.handle
-        lists = h.bin_lazy(self._prims(False), self.ext, tuple(self.templates.shape))
+        lists = h.bin_lazy(self._prims(False), self.ext, tuple(self.templates.shape), h.bin_factor)
         h.forward(self._prims(True), self._bm, lists, self.eps_T, self.color, self.alpha, self.T, self.last)
         bg = self.bg if self.bg is not None else torch.ones_like(self.color)
         return self.color + (1.0 - self.alpha).unsqueeze(-1) * bg

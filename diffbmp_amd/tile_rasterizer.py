@@ -139,7 +139,8 @@ class _Run:
         if mapping is not None and mapping.numel() > 0:
             self.lists = _native.Lists(*unpack_mapping(mapping, handle.tiles))
         else:
-            self.lists = handle.bin_lazy(self.prims(), self.ext, tuple(self.templates.shape))
+            self.lists = handle.bin_lazy(self.prims(), self.ext, tuple(self.templates.shape),
+                                         handle.bin_factor)
 
     def prims(self):
         return Handle.prims(self.means2D, self.radii, self.rotations, self.opacities, self.colors,

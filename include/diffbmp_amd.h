@@ -84,6 +84,9 @@ typedef struct {
   const int32_t* tile_indices;
   const int64_t* pair_count;
   int64_t capacity;
+  int32_t bin_factor;       /* 0 or 1: one list per tile; f > 1: lists from
+                               dbmp_bin_tiles, one per super-tile of f x f tiles */
+  const int32_t* ranges;    /* f > 1: (N,4) tile ranges written by dbmp_bin_tiles */
 } dbmp_tile_lists;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -120,6 +123,19 @@ int dbmp_template_extents(dbmp_rasterizer* h, const dbmp_bitmaps* bm, float* ext
 int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
              int tmpl_h, int tmpl_w, int32_t* tile_offsets, int32_t* tile_indices,
              int64_t capacity, int64_t* d_pair_count, void* stream);
+
+/* Binning into coarser lists for the raster calls of this handle: one list
+ * per super-tile of bin_factor x bin_factor tiles (bin_factor 2, 4 or 8; tile
+ * size 16), ascending primitive id, plus each primitive's tile range (N,4)
+ * int32 in ranges_out.  The raster kernels use an entry in a tile only inside
+ * that range, so forward and backward are exactly those of dbmp_bin's per-tile
+ * lists, with bin_factor^2 fewer pairs to sort.  tile_offsets holds
+ * ceil(tilesX/f)*ceil(tilesY/f)+1 entries.  Pass bin_factor and ranges_out on
+ * in dbmp_tile_lists. */
+int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,
+                   int tmpl_h, int tmpl_w, int32_t bin_factor, int32_t* tile_offsets,
+                   int32_t* tile_indices, int64_t capacity, int64_t* d_pair_count,
+                   int32_t* ranges_out, void* stream);
 
 /* Forward; replaces _C.rasterize_tiles_class / _C_fp16.rasterize_tiles_class_fp16
  * (ext.cpp:11-28 -> tile_rasterize.cu:297-498).  Outputs (fp32):
