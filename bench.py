@@ -369,12 +369,15 @@ def main():
     P, list_mean, walked = list_stats(res["handle"], sc, dev, sdt, eps_T)
 
     traffic = None
+    # the dominant kernel: the backward (with the fused composite/MSE prologue
+    # in the fused step)
+    bwd_kernel = "k_raster_bwd_mse" if args.mode == "fused" else "k_raster_bwd"
     prof = os.path.join(HERE, "profiles", "pmc_traffic_scale_fp32.json")
     if args.precision == "fp32" and os.path.exists(prof):
         # HBM bytes per backward launch from the committed FETCH_SIZE / WRITE_SIZE
         # passes of this workload, scaled by the copy-probe factors recorded there
         with open(prof) as f:
-            traffic = json.load(f)["kernels"]["k_raster_bwd"]["traffic_bytes"]
+            traffic = json.load(f)["kernels"].get(bwd_kernel, {}).get("traffic_bytes")
     b_p = torch.tensor([], dtype=sdt).element_size()
     b_bin, b_fwd, b_bwd = algorithmic_bytes(H, W, N, T, Ht, Wt, P, res["handle"].tiles, b_p, b_p)
     achieved = b_bwd / res["bwd_s"]
@@ -408,7 +411,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
                      "traffic_source": "profiles/pmc_traffic_scale_fp32.json" if traffic else None,
-                     "kernel": "k_raster_bwd", "algorithmic_bytes": int(b_bwd),
+                     "kernel": bwd_kernel, "algorithmic_bytes": int(b_bwd),
                      "step_algorithmic_bytes": int(b_bin + b_fwd + b_bwd)},
     }
     if args.psnr_steps:

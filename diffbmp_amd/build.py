@@ -16,6 +16,9 @@ HIPCC_FLAGS = [
     "-munsafe-fp-atomics",
     # no packed-fp32 instruction selection
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no SLP vectorisation: it packs the per-lane push predicates into
+    # small integer vectors (VALU shifts and compares) instead of lane masks
+    "-fno-slp-vectorize",
 ]
 
 

@@ -88,7 +88,7 @@ int check_job(const dbmp_job& j, int i) {
   if (!j.target || !j.out_color || !j.out_alpha || !j.out_T || !j.out_last)
     return fail(DBMP_E_ARG, "job %d: an image pointer is NULL", i);
   if (!j.loss_history || j.loss_capacity < 1) return fail(DBMP_E_ARG, "job %d: no loss history", i);
-  if (!(j.eps_T >= 0.f)) return fail(DBMP_E_ARG, "job %d: eps_T must be >= 0", i);
+  if (!(j.eps_T >= 0.f && j.eps_T < 1.f)) return fail(DBMP_E_ARG, "job %d: eps_T must be in [0, 1)", i);
   if (j.steps_done < 0) return fail(DBMP_E_ARG, "job %d: steps_done < 0", i);
   return 0;
 }

@@ -457,7 +457,7 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   if ((rc = check_bitmaps(bm))) return rc;
   if ((rc = check_lists(h, lists))) return rc;
   if (!out_color || !out_alpha || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL output");
-  if (!(eps_T >= 0.f)) return fail(DBMP_E_ARG, "eps_T must be >= 0");
+  if (!(eps_T >= 0.f && eps_T < 1.f)) return fail(DBMP_E_ARG, "eps_T must be in [0, 1)");
   hipStream_t st = (hipStream_t)stream;
   const int N = prims->num_primitives;
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;

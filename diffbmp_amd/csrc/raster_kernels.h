@@ -195,6 +195,13 @@ __device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
   return m;
 }
 
+// Lane-mask vote on a bool: the ballot stays a scalar mask (no per-lane 0/1
+// value is materialised first, as an int-predicate vote does).
+__device__ __forceinline__ unsigned long long wave_ballot(bool b) {
+  return __builtin_amdgcn_ballot_w64(b);
+}
+__device__ __forceinline__ bool wave_any(bool b) { return wave_ballot(b) != 0ull; }
+
 // Stage batch entry ri (one quarter per thread) into LDS slot `buf`.
 struct Stage {
   float4 v;
@@ -296,6 +303,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   // of zero factors (exactly opaque layers), capped at 2
   float T = 1.f, Tnz = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
   int nz = 0;
+  float thr = alive ? A.eps : -1.f;   // stop test threshold (below)
   int last = -1;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
@@ -308,9 +316,9 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     const bool have_next = base + kBatch < end;
     Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
-    if (__any(alive)) {
+    if (wave_any(alive)) {
       // entries of this batch whose push box can touch this wave's rows
-      unsigned long long wm = __ballot((smask[buf][G.lane] & wbit) != 0);
+      unsigned long long wm = wave_ballot((smask[buf][G.lane] & wbit) != 0);
       while (wm) {
         const int j = __builtin_ctzll(wm);
         wm &= wm - 1ull;
@@ -318,7 +326,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         const Sample<DT> S = eval_sample<DT, false, BLEND>(A, p, fpx, fpy, alive);
         if constexpr (BLEND) {
           // skip the colour-map gathers when nothing of this entry reaches the wave
-          if (!__any(S.pushed)) continue;
+          if (!wave_any(S.pushed)) continue;
         }
         float col[3];
         if (BLEND && A.cmap != nullptr) {
@@ -335,14 +343,27 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         C2 = fmaf(wgt, col[2], C2);
         const float f = fmaxf(1.f - a, 0.f);
         T = T * f;
-        Tnz = f > 0.f ? Tnz * f : Tnz;
-        nz = f > 0.f ? nz : min(nz + 1, 2);   // exactly opaque layers, capped at 2
         // a pixel stops once later layers can change neither its colour nor
         // its alpha (or alpha's derivative w.r.t. an opaque layer) by eps_T
-        const bool dead = S.pushed & ((Tnz < A.eps) | (nz >= 2));
-        last = dead ? base - start + j + 1 : last;
-        alive = alive & !dead;
-        if (!__any(alive)) break;
+        // (Tnz < thr: thr is eps_T on a live lane, +inf once it has seen two
+        // opaque layers, -1 on a stopped lane; a live lane that is not pushed
+        // keeps Tnz >= eps_T, since eps_T < 1, so the test needs no `pushed`)
+        // Tnz keeps the nonzero factors; nz counts exactly opaque layers
+        // (capped at 2), rare enough for a wave-uniform branch
+        const bool opq = f == 0.f;
+        Tnz = opq ? Tnz : Tnz * f;
+        if (wave_any(opq)) {
+          __asm__ volatile("");   // keeps this rare block a branch, not selects
+          nz = opq ? min(nz + 1, 2) : nz;
+          thr = (alive & (nz >= 2)) ? __builtin_inff() : thr;
+        }
+        const bool dead = Tnz < thr;
+        if (wave_any(dead)) {
+          last = dead ? base - start + j + 1 : last;
+          alive = alive & !dead;
+          thr = dead ? -1.f : thr;
+          if (!wave_any(alive)) break;
+        }
       }
     }
     if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G.bx, G.by);
@@ -434,6 +455,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     ga = (okpix && gAin) ? gAin[pix] : 0.f;
   }
   float T = 1.f, ca0 = 0.f, ca1 = 0.f, ca2 = 0.f;
+  const float hyx = A.hsy / A.hsx;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
   const int wkmax = __builtin_amdgcn_readfirstlane(wave_max_i32(lst));
@@ -454,7 +476,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     const int cnt = __builtin_amdgcn_readfirstlane(min(kBatch, stop - base));
     // entries of this batch that touch this wave's rows, up to the wave's
     // furthest forward position
-    unsigned long long wm = __ballot((smask[buf][G.lane] & wbit) != 0);
+    unsigned long long wm = wave_ballot((smask[buf][G.lane] & wbit) != 0);
     const int lim = wkmax - (base - start);
     wm = lim >= 64 ? wm : (lim <= 0 ? 0ull : wm & ((1ull << lim) - 1ull));
     while (wm) {
@@ -463,7 +485,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const int kk = base - start + j;
       const PrimRec p = srec[buf][j];
       const Sample<DT> S = eval_sample<DT, true, BLEND>(A, p, fpx, fpy, kk < lst);
-      if (!__any(S.pushed)) continue;
+      if (!wave_any(S.pushed)) continue;
       float col[3], dcx[3] = {0.f, 0.f, 0.f}, dcy[3] = {0.f, 0.f, 0.f};
       const bool has_map = BLEND && A.cmap != nullptr;
       if (has_map) {
@@ -481,16 +503,22 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       ca2 = fmaf(wgt, col[2], ca2);
       const float S0 = cf0 - ca0, S1 = cf1 - ca1, S2 = cf2 - ca2;
       const float om = 1.f - a;
-      const float inv1m = __builtin_amdgcn_rcpf(fmaxf(om, 1e-6f));
+      // max(om, 1e-6) as one med3 (om <= 1 - 0 for a >= 0; FLT_MAX bounds it otherwise)
+      const float inv1m = __builtin_amdgcn_rcpf(__builtin_amdgcn_fmed3f(om, 1e-6f, 3.402823466e38f));
       const float fk = fmaxf(om, 0.f);
       const float Tn = Tk * fk;
-      // suffix transmittance B_k = prod_{m>k} (1 - a_m) = T_final / T_{k+1};
-      // for an exactly opaque layer (fk = 0) it is the nonzero-factor product
-      // after it, tnzf / T_k, if it is the only opaque layer, else 0
-      // (selects, one reciprocal per lane)
+      // suffix transmittance B_k = prod_{m>k} (1 - a_m) = T_final / T_{k+1},
+      // 1 once T is 0; for an exactly opaque layer (fk = 0) it is the
+      // nonzero-factor product after it, tnzf / T_k, if it is the only opaque
+      // layer, else 0 (selects, one reciprocal per lane)
       const bool tpos = Tn > 0.f;
-      const float rB = fminf((tpos ? tf : tnzf) * __builtin_amdgcn_rcpf(tpos ? Tn : Tk), 1.f);
-      const float B = tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f);
+      float B;
+      if (wave_any(fk == 0.f)) {
+        const float rB = fminf((tpos ? tf : tnzf) * __builtin_amdgcn_rcpf(tpos ? Tn : Tk), 1.f);
+        B = tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f);
+      } else {
+        B = tpos ? fminf(tf * __builtin_amdgcn_rcpf(Tn), 1.f) : 1.f;
+      }
       float dLda = ga * (Tk * B);
       dLda = fmaf(gc0, fmaf(Tk, col[0], -S0 * inv1m), dLda);
       dLda = fmaf(gc1, fmaf(Tk, col[1], -S1 * inv1m), dLda);
@@ -510,14 +538,16 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       acc[4] = dLda * me;
       // mask path (:314-343)
       const float dLdm = S.pushed ? dLda * p.abase : 0.f;
-      float dLdu = dLdm * (S.gxt * A.hsx);
-      float dLdv = dLdm * (S.gyt * A.hsy);
+      // dL/du and dL/dv divided by hsx = dtx/du (hsx is applied with the
+      // per-primitive factors); hyx = hsy / hsx
+      float dLdu = dLdm * S.gxt;
+      float dLdv = dLdm * (S.gyt * hyx);
       if constexpr (BLEND) {
         if (exact) {
           // colour-map term: dC/dcol_k = T_k a_k, dcol/d(tx,ty) = c_blend dc_o/d(tx,ty)
           const float kw = wgt * A.c_blend;
-          dLdu = fmaf(kw * A.hsx, fmaf(gc0, dcx[0], fmaf(gc1, dcx[1], gc2 * dcx[2])), dLdu);
-          dLdv = fmaf(kw * A.hsy, fmaf(gc0, dcy[0], fmaf(gc1, dcy[1], gc2 * dcy[2])), dLdv);
+          dLdu = fmaf(kw, fmaf(gc0, dcx[0], fmaf(gc1, dcx[1], gc2 * dcx[2])), dLdu);
+          dLdv = fmaf(kw * hyx, fmaf(gc0, dcy[0], fmaf(gc1, dcy[1], gc2 * dcy[2])), dLdv);
         }
       }
       // (u, v) = R(theta) (dx, dy) / r: with (gx, gy) = R^T (dL/du, dL/dv),
@@ -543,7 +573,8 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const int j = i >> 3, q = i & 7;
       const PrimRec& pr = srec[buf][j];
       const float cc = q == 5 ? pr.cr : (q == 6 ? pr.cg : pr.cb);
-      const float fac = q < 3 ? -pr.rinv : (q == 3 ? 1.f : (q == 4 ? pr.dav : cc * (1.f - cc) * cwf));
+      const float fac = q < 3 ? -pr.rinv * A.hsx
+                              : (q == 3 ? A.hsx : (q == 4 ? pr.dav : cc * (1.f - cc) * cwf));
       const float s = (sred[0][j][q] + sred[1][j][q] + sred[2][j][q] + sred[3][j][q]) * fac;
       sred[0][j][q] = 0.f;
       sred[1][j][q] = 0.f;

@@ -143,9 +143,9 @@ int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext
  *   out_T_final (H,W) and out_last (H,W) int32: per-pixel state dbmp_backward
  *   needs -- the product of the pixel's nonzero (1 - alpha) factors, and the
  *   number of list entries consumed in bits 0..29 with the count of exactly
- *   opaque layers (0, 1, 2+) in bits 30..31.  eps_T: a pixel stops compositing
- *   once later layers can change it by less than eps_T (0 = walk the whole
- *   list). */
+ *   opaque layers (0, 1, 2+) in bits 30..31.  eps_T in [0, 1): a pixel stops
+ *   compositing once later layers can change it by less than eps_T (0 = walk
+ *   the whole list). */
 int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
                  const dbmp_tile_lists* lists, float eps_T, float* out_color, float* out_alpha,
                  float* out_T_final, int32_t* out_last, void* stream);
