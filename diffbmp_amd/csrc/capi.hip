@@ -65,9 +65,16 @@ struct dbmp_rasterizer {
   int hist_blocks = 0;
   int* hist = nullptr;
   int* totals = nullptr;
+  // per-block forward costs and the heaviest-first block order derived from
+  // them (valid for order_n blocks; 0: none yet)
+  int blk_cap = 0;
+  int* blk_cost = nullptr;
+  int* blk_order = nullptr;
+  int order_n = 0;
   // texel quads cache
   void* quads = nullptr;
-  size_t quads_bytes = 0;
+  size_t quads_bytes = 0;     // allocated
+  size_t quads_used = 0;      // bytes of the current bitmaps' records
   const void* q_src = nullptr;
   long long q_version = -2;
   int q_T = 0, q_Ht = 0, q_Wt = 0, q_dt = -1;
@@ -109,6 +116,27 @@ int ensure_pairs(dbmp_rasterizer* h, long long cap) {
   h->pair_cap = cap;
   h->hist_blocks = nblk;
   return 0;
+}
+
+// Per-block cost and order buffers for nb blocks.
+int ensure_blocks(dbmp_rasterizer* h, int nb) {
+  if (nb <= h->blk_cap) return 0;
+  int rc;
+  if ((rc = grow(&h->blk_cost, nb))) return rc;
+  if ((rc = grow(&h->blk_order, nb))) return rc;
+  h->blk_cap = nb;
+  h->order_n = 0;
+  return 0;
+}
+
+// Raster launches of nb blocks run heaviest first once a forward of nb
+// blocks has measured the costs -- when the bitmaps fit in one XCD's 4 MiB
+// L2.  Larger bitmaps are L2-bound: there the natural order, which runs
+// neighbouring tiles (sharing texels) together, is faster.
+constexpr size_t kOrderMaxQuadBytes = 4u << 20;
+bool uses_block_order(const dbmp_rasterizer* h) { return h->quads_used <= kOrderMaxQuadBytes; }
+const int* block_order(const dbmp_rasterizer* h, int nb) {
+  return (h->order_n == nb && uses_block_order(h)) ? h->blk_order : nullptr;
 }
 
 int check_prims(const dbmp_prims* p) {
@@ -197,6 +225,7 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   h->q_rgba = rgba;
   h->q_csrc = b->colors_orig;
   h->q_cversion = b->colors_version;
+  h->quads_used = (size_t)tot * esz;
   return 0;
 }
 
@@ -216,6 +245,8 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.cmap_mode = (blend && b->colors_orig_mode == 2) ? 2 : 1;
   A.ranges = nullptr;
   A.bin_shift = 0;
+  A.order = nullptr;
+  A.cost = nullptr;
   A.c_blend = b->c_blend;
   A.exact_blend = b->blend_grad_exact != 0;
   A.N = p->num_primitives;
@@ -336,7 +367,7 @@ int dbmp_destroy(dbmp_rasterizer* h) {
   timing_drain(h->ev_fwd, ms, n);
   timing_drain(h->ev_bwd, ms, n);
   void* ptrs[] = {h->recs, h->counts, h->offs, h->ranges, h->grad8, h->keysA,
-                  h->keysB, h->valsB, h->hist, h->totals, h->quads};
+                  h->keysB, h->valsB, h->hist, h->totals, h->quads, h->blk_cost, h->blk_order};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -465,11 +496,23 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   if ((rc = ensure_quads(h, bm, st))) return rc;
   // raster constants only (no extents => no tile ranges)
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
-  const RasterArgs A = make_args(h, prims, bm, lists, eps_T);
+  RasterArgs A = make_args(h, prims, bm, lists, eps_T);
+  const int nb = A.tiles * A.subs;
+  if ((rc = ensure_blocks(h, nb))) return rc;
+  const bool ordered = uses_block_order(h);
+  A.order = block_order(h, nb);
+  A.cost = ordered ? h->blk_cost : nullptr;
   const bool blend = A.cmap != nullptr;
-  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(A.tiles * A.subs), dim3(kThreads), 0, st, A,
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(nb), dim3(kThreads), 0, st, A,
                     out_color, out_alpha, out_T_final, out_last);
   LAUNCH_CHECK("k_raster_fwd");
+  if (ordered) {
+    // this forward's costs order the backward and the next forward
+    hipLaunchKernelGGL(k_block_order, dim3(1), dim3(kOrderThreads), 0, st, h->blk_cost, nb,
+                       h->blk_order);
+    LAUNCH_CHECK("k_block_order");
+    h->order_n = nb;
+  }
   timing_end(h, h->ev_fwd, st);
   return 0;
 }
@@ -498,7 +541,8 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
     hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
     LAUNCH_CHECK("k_fill_f32");
   }
-  const RasterArgs A = make_args(h, prims, bm, lists, 0.f);
+  RasterArgs A = make_args(h, prims, bm, lists, 0.f);
+  A.order = block_order(h, A.tiles * A.subs);
   const bool blend = A.cmap != nullptr;
   DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd, dim3(A.tiles * A.subs), dim3(kThreads), 0, st, A,
                     grad_color, grad_alpha, out_color, out_T_final, out_last, h->grad8);
@@ -537,7 +581,8 @@ int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bi
     hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
     LAUNCH_CHECK("k_fill_f32");
   }
-  const RasterArgs A = make_args(h, prims, bm, lists, 0.f);
+  RasterArgs A = make_args(h, prims, bm, lists, 0.f);
+  A.order = block_order(h, A.tiles * A.subs);
   LossArgs L;
   L.target = target;
   L.bg = background;

@@ -52,7 +52,16 @@ struct RasterArgs {
   int H, W, ts, tilesX, tiles, subs;   // ts: list tile; subs = (ts/16)^2 blocks per tile
   float eps;
   int bin_shift;         // log2(list tile / 16) when ranges is set
+  const int* order;      // workgroup -> block, heaviest first (k_block_order), or null
+  int* cost;             // forward: each block's walk cost, or null
 };
+
+// Block a workgroup runs: the heaviest-first order of the last forward's
+// costs when there is one, else the XCD-interleaved natural order.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks);
+__device__ __forceinline__ int block_of(const RasterArgs& A, int wg) {
+  return A.order != nullptr ? A.order[wg] : xcd_remap(wg, A.tiles * A.subs);
+}
 
 // blockIdx -> (tile, sub-tile).  Consecutive workgroups are dealt round-robin
 // to the 8 XCDs; remap so that each XCD gets a contiguous band of tiles and
@@ -296,7 +305,10 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   __shared__ int sid[2][kBatch];
   __shared__ int smask[2][kBatch];
   const BlockGeom G = block_geom(A, b);
-  if (G.bx >= A.W || G.by >= A.H) return;   // 16-px block of a tile past the canvas edge
+  if (G.bx >= A.W || G.by >= A.H) {   // 16-px block of a tile past the canvas edge
+    if (A.cost != nullptr && threadIdx.x == 0) A.cost[b] = 0;
+    return;
+  }
   const float fpx = (float)G.px, fpy = (float)G.py;
   bool alive = (G.px < A.W) && (G.py < A.H);
   // T: transmittance; Tnz: product of the nonzero (1 - a) factors; nz: count
@@ -305,6 +317,8 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   int nz = 0;
   float thr = alive ? A.eps : -1.f;   // stop test threshold (below)
   int last = -1;
+  int walked = 0;   // entries this wave evaluated (wave-uniform)
+  int batches = 0;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
@@ -315,6 +329,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   for (int base = start; base < end; base += kBatch, buf ^= 1) {
     const bool have_next = base + kBatch < end;
     Stage nxt;
+    ++batches;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
     if (wave_any(alive)) {
       // entries of this batch whose push box can touch this wave's rows
@@ -322,6 +337,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
       while (wm) {
         const int j = __builtin_ctzll(wm);
         wm &= wm - 1ull;
+        ++walked;
         const PrimRec p = srec[buf][j];
         const Sample<DT> S = eval_sample<DT, false, BLEND>(A, p, fpx, fpy, alive);
         if constexpr (BLEND) {
@@ -378,6 +394,14 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     out_T[p] = Tnz;
     out_last[p] = (last < 0 ? end - start : last) | (nz << 30);
   }
+  if (A.cost != nullptr) {
+    // block cost: the slowest wave's entry count plus the staging of its batches
+    __syncthreads();
+    if (G.lane == 0) smask[0][G.w] = walked + 8 * batches;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      A.cost[b] = max(max(smask[0][0], smask[0][1]), max(smask[0][2], smask[0][3]));
+  }
 }
 
 template <int DT, bool BLEND>
@@ -385,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __
                                                          float* __restrict__ out_alpha,
                                                          float* __restrict__ out_T,
                                                          int* __restrict__ out_last) {
-  raster_fwd_block<DT, BLEND>(A, xcd_remap(blockIdx.x, A.tiles * A.subs), out_color, out_alpha,
+  raster_fwd_block<DT, BLEND>(A, block_of(A, blockIdx.x), out_color, out_alpha,
                               out_T, out_last);
 }
 
@@ -606,7 +630,7 @@ static __global__ __launch_bounds__(kThreads) void k_raster_bwd_mse(RasterArgs A
                                                                     const float* __restrict__ Tfin,
                                                                     const int* __restrict__ lastin,
                                                                     float* __restrict__ grad8) {
-  raster_bwd_block<DT, BLEND, true>(A, xcd_remap(blockIdx.x, A.tiles * A.subs), nullptr, nullptr, L,
+  raster_bwd_block<DT, BLEND, true>(A, block_of(A, blockIdx.x), nullptr, nullptr, L,
                                     Cfin, Tfin, lastin, grad8);
 }
 
@@ -618,8 +642,43 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
                                                          const int* __restrict__ lastin,
                                                          float* __restrict__ grad8) {
   const LossArgs none{};
-  raster_bwd_block<DT, BLEND, false>(A, xcd_remap(blockIdx.x, A.tiles * A.subs), gCin, gAin,
+  raster_bwd_block<DT, BLEND, false>(A, block_of(A, blockIdx.x), gCin, gAin,
                                      none, Cfin, Tfin, lastin, grad8);
+}
+
+// Heaviest-first order of nb blocks from their forward costs: 256 cost
+// buckets (bucket 0 the heaviest), blocks within a bucket in any order.
+// One workgroup.  Launching the heavy blocks first keeps the last workgroups
+// of a launch short, so the machine stays full until the end.
+constexpr int kOrderThreads = 1024;
+static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int* __restrict__ cost,
+                                                                    int nb, int* __restrict__ order) {
+  __shared__ int cnt[256];
+  __shared__ int smax;
+  if (threadIdx.x == 0) smax = 1;
+  if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int m = 1;
+  for (int i = threadIdx.x; i < nb; i += kOrderThreads) m = max(m, cost[i]);
+  atomicMax(&smax, m);
+  __syncthreads();
+  const long long mx = smax;
+  auto bucket = [&](int c) {
+    const long long q = ((long long)max(c, 0) * 255 + mx - 1) / mx;   // 0..255, ascending cost
+    return 255 - (int)min(q, 255LL);
+  };
+  for (int i = threadIdx.x; i < nb; i += kOrderThreads) atomicAdd(&cnt[bucket(cost[i])], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < 256; ++k) {
+      const int c = cnt[k];
+      cnt[k] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += kOrderThreads) order[atomicAdd(&cnt[bucket(cost[i])], 1)] = i;
 }
 
 }  // namespace dbmp
