@@ -23,6 +23,7 @@
 //     suffix colour as C_final - C_prefix and the suffix transmittance as
 //     T_final / T_{k+1}.
 #pragma once
+#include "bin_kernels.h"
 #include "dbmp_device.h"
 
 namespace dbmp {
@@ -648,35 +649,30 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
 
 // Heaviest-first order of nb blocks from their forward costs: 256 cost
 // buckets (bucket 0 the heaviest), blocks within a bucket in any order.
-// One workgroup.  Launching the heavy blocks first keeps the last workgroups
-// of a launch short, so the machine stays full until the end.
-constexpr int kOrderThreads = 1024;
+// One 256-thread workgroup.  Launching the heavy blocks first keeps the last
+// workgroups of a launch short, so the machine stays full until the end.
+constexpr int kOrderThreads = 256;
 static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int* __restrict__ cost,
                                                                     int nb, int* __restrict__ order) {
   __shared__ int cnt[256];
-  __shared__ int smax;
-  if (threadIdx.x == 0) smax = 1;
-  if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
-  __syncthreads();
+  __shared__ int wmax[4];
+  __shared__ long long sh[4];
+  cnt[threadIdx.x] = 0;
   int m = 1;
   for (int i = threadIdx.x; i < nb; i += kOrderThreads) m = max(m, cost[i]);
-  atomicMax(&smax, m);
+  m = wave_max_i32(m);
+  if (lane_id() == 0) wmax[threadIdx.x >> 6] = m;
   __syncthreads();
-  const long long mx = smax;
+  const long long mx = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
   auto bucket = [&](int c) {
     const long long q = ((long long)max(c, 0) * 255 + mx - 1) / mx;   // 0..255, ascending cost
     return 255 - (int)min(q, 255LL);
   };
   for (int i = threadIdx.x; i < nb; i += kOrderThreads) atomicAdd(&cnt[bucket(cost[i])], 1);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int k = 0; k < 256; ++k) {
-      const int c = cnt[k];
-      cnt[k] = acc;
-      acc += c;
-    }
-  }
+  long long tot;
+  const int ex = (int)block256_excl_scan(cnt[threadIdx.x], tot, sh);
+  cnt[threadIdx.x] = ex;
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kOrderThreads) order[atomicAdd(&cnt[bucket(cost[i])], 1)] = i;
 }
