@@ -99,7 +99,10 @@ const char* dbmp_last_error(void);
  * tile_size must be a positive multiple of 16.  sigma and
  * max_prims_per_pixel are accepted for signature parity and not used: the
  * reference never reads sigma in its kernels, and this implementation keeps
- * no per-pixel layer cache, so it has no per-pixel capacity limit. */
+ * no per-pixel layer cache, so it has no per-pixel capacity limit.  A
+ * handle's scratch (primitive records, sort buffers, texel records, block
+ * costs and launch order) is shared by its calls: use a handle from one
+ * stream at a time. */
 int dbmp_init(dbmp_rasterizer** out, int image_height, int image_width, int tile_size,
               float sigma, float alpha_upper_bound, int max_prims_per_pixel, int num_primitives);
 
