@@ -79,6 +79,14 @@ class Job(ctypes.Structure):
                 ("clamp", ctypes.c_int32), ("steps_done", ctypes.c_int32)]
 
 
+class LayerPrims(ctypes.Structure):
+    """``dbmp_layer_prims``: the layer export's primitive arrays (fp32)."""
+    _fields_ = [("means2D", ctypes.c_void_p), ("radii", ctypes.c_void_p),
+                ("rotations", ctypes.c_void_p), ("colors", ctypes.c_void_p),
+                ("visibility", ctypes.c_void_p), ("bmp_sel", ctypes.c_void_p),
+                ("num_primitives", ctypes.c_int32)]
+
+
 class Lists:
     """Tile lists on the device: CSR offsets, index buffer, and -- for lists
     produced by dbmp_bin -- the device pair count and the capacity used.
@@ -154,6 +162,15 @@ EXPORTS = {
                                            ctypes.c_int64, ctypes.c_void_p]),
     "dbmp_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "dbmp_batch_pair_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "dbmp_psd_bounds": (ctypes.c_int, [ctypes.POINTER(LayerPrims), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_psd_layers": (ctypes.c_int, [ctypes.POINTER(LayerPrims), ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
 }
 
 

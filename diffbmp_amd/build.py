@@ -5,7 +5,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("capi.hip", "batch.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("capi.hip", "batch.hip", "psd.hip")]
 OUT = os.path.join(HERE, "libdiffbmp_amd.so")
 
 HIPCC_FLAGS = [

@@ -298,6 +298,47 @@ int dbmp_batch_step(dbmp_batch* b, int32_t use_graph, void* stream);
 /* Pair count of the most recent step that has completed on the device. */
 int dbmp_batch_pair_count(dbmp_batch* b, int64_t* pairs);
 
+/* ---- Layer export: per-primitive uint8 RGBA crops ------------------------
+ * Replaces the reference's layer-export extension (psd_export_cuda:
+ * psd_export_bindings.cpp:38-129 -> psd_export_kernels.cu:12-275) and the
+ * host prefix sum between its two stages (psd_export_uint8.py:64-75).
+ * All arrays fp32 except bmp_sel; colors and visibility have the sigmoid
+ * applied already (psd_exporter.py:80-81). */
+typedef struct {
+  const float* means2D;     /* (N,2) */
+  const float* radii;       /* (N) */
+  const float* rotations;   /* (N) */
+  const float* colors;      /* (N,3) */
+  const float* visibility;  /* (N) */
+  const int32_t* bmp_sel;   /* (N) */
+  int32_t num_primitives;
+} dbmp_layer_prims;
+
+/* Stage 1 (compute_bounding_boxes_kernel, psd_export_kernels.cu:12-55):
+ * crop box (left, top, right, bottom) (N,4) int32 of a 1.5 r square around
+ * the centre scaled by scale_factor, clipped to the canvas ((0,0,1,1) when
+ * empty); its size (w, h) (N,2) int32; byte_offsets (N+1) int64, the offset
+ * of each layer's w*h*4 RGBA bytes in one packed buffer, and the total in
+ * *d_total_bytes (device).  Reads means2D and radii only.  The reference
+ * kept int32 offsets; here they are 64-bit. */
+int dbmp_psd_bounds(const dbmp_layer_prims* prims, int canvas_height, int canvas_width,
+                    float scale_factor, int32_t* boxes, int32_t* sizes, int64_t* byte_offsets,
+                    int64_t* d_total_bytes, void* stream);
+
+/* Stage 2 (generate_cropped_layers_kernel, psd_export_kernels.cu:61-198):
+ * the packed RGBA crops into out (total_bytes, 16-byte aligned).  Colour
+ * maps (Ht,Wt,3 fp32 each): color_map_mode 0 none, 1 per primitive (N maps,
+ * the reference's gathered layout), 2 per template (T maps, indexed by
+ * bmp_sel); blended as (1 - c_blend) c + c_blend c_o when c_blend > 0.
+ * layer_alpha_max (N) int32 or NULL: each layer's largest alpha byte.
+ * Arithmetic is the reference's, in IEEE fp32 with correctly rounded
+ * cos/sin and division (the reference builds with fast math). */
+int dbmp_psd_layers(const dbmp_layer_prims* prims, const float* templates, int num_templates,
+                    int tmpl_h, int tmpl_w, const float* color_maps, int color_map_mode, float c_blend,
+                    int canvas_height, int canvas_width, float scale_factor, float alpha_upper_bound,
+                    const int32_t* boxes, const int64_t* byte_offsets, int64_t total_bytes,
+                    uint8_t* out, int32_t* layer_alpha_max, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

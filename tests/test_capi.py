@@ -57,7 +57,8 @@ def test_struct_layouts_match_header():
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     for cname, py in (("dbmp_prims", _native.Prims), ("dbmp_bitmaps", _native.Bitmaps),
                       ("dbmp_tile_lists", _native.TileLists), ("dbmp_adam_group", _native.AdamGroup),
-                      ("dbmp_adam", _native.Adam), ("dbmp_job", _native.Job)):
+                      ("dbmp_adam", _native.Adam), ("dbmp_job", _native.Job),
+                      ("dbmp_layer_prims", _native.LayerPrims)):
         body = re.search(r"typedef struct \{([^{}]*)\}\s*" + cname + ";", text).group(1)
         decls = [d for d in body.split(";") if d.strip()]
         n = sum(len(d.split(",")) for d in decls)
@@ -73,7 +74,8 @@ def test_struct_offsets_match_c_compiler(tmp_path):
         pytest.skip("gcc not available")
     structs = {"dbmp_prims": _native.Prims, "dbmp_bitmaps": _native.Bitmaps,
                "dbmp_tile_lists": _native.TileLists, "dbmp_adam_group": _native.AdamGroup,
-               "dbmp_adam": _native.Adam, "dbmp_job": _native.Job}
+               "dbmp_adam": _native.Adam, "dbmp_job": _native.Job,
+               "dbmp_layer_prims": _native.LayerPrims}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
