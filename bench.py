@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--psnr-steps", type=int, default=0,
                     help="also fit this many Adam steps at fp32 and at --precision from the same "
                          "start and report both final PSNRs")
-    ap.add_argument("--workload", default="scale", choices=["scale", "stress", "serving"],
+    ap.add_argument("--workload", default="scale", choices=["scale", "stress", "serving", "export"],
                     help="scale: the headline fit step; stress: the 2048^2 / 32768-primitive "
                          "RGBA colour-mapped fit step; serving: a Poisson stream of fit jobs "
                          "through the batched multi-job engine (each its own JSON line)")
@@ -333,6 +333,60 @@ def run_serving(args, dev):
     print(json.dumps(out))
 
 
+def run_export(args, dev):
+    """Layer export of the scale scene: the uint8 RGBA crop (1.5 r square) of
+    every primitive into one packed buffer; HBM-write bound."""
+    import ctypes
+    import torch
+    from diffbmp_amd import _native
+    from diffbmp_amd import scenes as S
+    from diffbmp_amd.psd_export_uint8 import layer_bounds
+
+    sc = S.make_scene(1)
+    N = sc.N
+    rng = np.random.default_rng(5)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    md, rd, thd, seld, tmd = t(sc.means2D), t(sc.radii), t(sc.rotations), t(sc.bmp_sel), t(sc.templates)
+    cd = t(rng.uniform(0, 1, (N, 3)).astype(np.float32))
+    vd = t(rng.uniform(0, 1, N).astype(np.float32))
+    boxes, sizes, offs, total = layer_bounds(md, rd, sc.H, sc.W, 1.0)
+    buf = torch.empty(total, dtype=torch.uint8, device=dev)
+    amax = torch.empty(N, dtype=torch.int32, device=dev)
+    p = _native.LayerPrims(md.data_ptr(), rd.data_ptr(), thd.data_ptr(), cd.data_ptr(), vd.data_ptr(),
+                           seld.data_ptr(), N)
+    T, Ht, Wt = sc.templates.shape
+    st = torch.cuda.current_stream(dev)
+    call = lambda: _native.check(_native.lib().dbmp_psd_layers(
+        ctypes.byref(p), tmd.data_ptr(), T, Ht, Wt, None, 0, 0.0, sc.H, sc.W, 1.0, 1.0,
+        boxes.data_ptr(), offs.data_ptr(), total, buf.data_ptr(), amax.data_ptr(), _native._stream(dev)))
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(args.steps):
+        call()
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    px = total // 4
+    # algorithmic bytes: the crops written once, the per-layer constants and
+    # the templates read once
+    algo = total + N * 40 + T * Ht * Wt * 4
+    out = {
+        "metric": "layer export GB/s (uint8 RGBA crops, 1024^2, 8192 prims)",
+        "value": round(total / ms / 1e6, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "layer export: scale scene (1024x1024, 8192 primitives, 4x128^2 bitmaps), "
+                               "1.5 r crops, scale 1, no colour maps", "layers": N, "pixels": int(px)},
+        "roofline": {"bound": "hbm", "achieved": round(algo / ms / 1e6, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(algo / ms / 1e6 / (HBM_PEAK / 1e9), 5),
+                     "traffic": None, "kernel": "k_psd_layers", "algorithmic_bytes": int(algo)},
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -349,6 +403,9 @@ def main():
         return
     if args.workload == "stress":
         run_stress(args, dev)
+        return
+    if args.workload == "export":
+        run_export(args, dev)
         return
     if args.copy_probe:
         nb = args.copy_probe << 20

@@ -18,7 +18,7 @@
 namespace dbmp {
 
 constexpr int kRadixThreads = 256;          // 4 waves
-constexpr int kRadixItems = 16;             // keys per thread
+constexpr int kRadixItems = 4;              // keys per thread (1024-key blocks: many small blocks)
 constexpr int kRadixTile = kRadixThreads * kRadixItems;  // keys per block
 
 // Template extents (vector_renderer.py:151-192): min/max column/row of texels

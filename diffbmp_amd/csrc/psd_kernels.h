@@ -235,46 +235,61 @@ static __global__ __launch_bounds__(kPsdThreads) void k_psd_layers(PsdArgs P, co
   }
   __syncthreads();
   const long long g = g0 + 4LL * threadIdx.x;
-  if (g > gl) return;
+  const bool active = g <= gl;   // no early return: the wave reduction below needs every lane
   unsigned v[4] = {0u, 0u, 0u, 0u};
-  int n = psd_layer_of(offs, g, nlo, nhi);
-  const long long p0 = offs[n] >> 2;
-  if (g + 3 <= gl && (offs[n + 1] >> 2) > g + 3) {
-    // the common case: all four pixels in one layer; one division for the
-    // row, then step along it
-    const PsdLayer L = psd_layer(P, boxes, n, scos[n - nlo], ssin[n - nlo]);
-    const long long loc = g - p0;
-    int ly = (int)(loc / L.w);
-    int lx = (int)(loc - (long long)ly * L.w);
-    int amax = 0;
+  int my_n = -1, my_amax = 0;    // layer and alpha maximum for the wave reduction
+  if (active) {
+    int n = psd_layer_of(offs, g, nlo, nhi);
+    const long long p0 = offs[n] >> 2;
+    if (g + 3 <= gl && (offs[n + 1] >> 2) > g + 3) {
+      // the common case: all four pixels in one layer; one division for the
+      // row, then step along it
+      const PsdLayer L = psd_layer(P, boxes, n, scos[n - nlo], ssin[n - nlo]);
+      const long long loc = g - p0;
+      int ly = (int)(loc / L.w);
+      int lx = (int)(loc - (long long)ly * L.w);
+      int amax = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      v[q] = psd_shade(P, L, L.left + lx, L.top + ly);
-      amax = max(amax, (int)(v[q] >> 24));
-      if (++lx == L.w) {
-        lx = 0;
-        ++ly;
+      for (int q = 0; q < 4; ++q) {
+        v[q] = psd_shade(P, L, L.left + lx, L.top + ly);
+        amax = max(amax, (int)(v[q] >> 24));
+        if (++lx == L.w) {
+          lx = 0;
+          ++ly;
+        }
+      }
+      my_n = n;
+      my_amax = amax;
+    } else {
+      // pixels of several layers (crops of a few pixels) or the buffer's end
+      for (int q = 0; q < 4; ++q) {
+        const long long gq = g + q;
+        if (gq > gl) break;
+        while (n < nhi && (offs[n + 1] >> 2) <= gq) ++n;
+        const PsdLayer L = psd_layer(P, boxes, n, scos[n - nlo], ssin[n - nlo]);
+        const long long loc = gq - (offs[n] >> 2);
+        const int ly = (int)(loc / L.w);
+        const int lx = (int)(loc - (long long)ly * L.w);
+        v[q] = psd_shade(P, L, L.left + lx, L.top + ly);
+        if (alpha_max != nullptr && (v[q] >> 24) != 0u) atomicMax(&alpha_max[n], (int)(v[q] >> 24));
       }
     }
-    if (alpha_max != nullptr && amax > 0) atomicMax(&alpha_max[n], amax);
-  } else {
-    // pixels of several layers (crops of a few pixels) or the buffer's end
-    for (int q = 0; q < 4; ++q) {
-      const long long gq = g + q;
-      if (gq > gl) break;
-      while (n < nhi && (offs[n + 1] >> 2) <= gq) ++n;
-      const PsdLayer L = psd_layer(P, boxes, n, scos[n - nlo], ssin[n - nlo]);
-      const long long loc = gq - (offs[n] >> 2);
-      const int ly = (int)(loc / L.w);
-      const int lx = (int)(loc - (long long)ly * L.w);
-      v[q] = psd_shade(P, L, L.left + lx, L.top + ly);
-      if (alpha_max != nullptr && (v[q] >> 24) != 0u) atomicMax(&alpha_max[n], (int)(v[q] >> 24));
+    if (g + 3 <= gl) {
+      *reinterpret_cast<uint4*>(out + g) = make_uint4(v[0], v[1], v[2], v[3]);
+    } else {
+      for (int q = 0; g + q <= gl; ++q) out[g + q] = v[q];
     }
   }
-  if (g + 3 <= gl) {
-    *reinterpret_cast<uint4*>(out + g) = make_uint4(v[0], v[1], v[2], v[3]);
-  } else {
-    for (int q = 0; g + q <= gl; ++q) out[g + q] = v[q];
+  if (alpha_max != nullptr) {
+    // one atomic per wave when all its fast-path lanes share one layer (the
+    // hot case: large crops), else one per lane
+    const int n0 = __builtin_amdgcn_readfirstlane(my_n);
+    if (n0 >= 0 && __builtin_amdgcn_ballot_w64(my_n >= 0 && my_n != n0) == 0ull) {
+      const int wm = wave_max_i32(my_amax);
+      if (lane_id() == 0 && wm > 0) atomicMax(&alpha_max[n0], wm);
+    } else if (my_amax > 0) {
+      atomicMax(&alpha_max[my_n], my_amax);
+    }
   }
 }
 
