@@ -12,6 +12,12 @@
 //   k_radix_*     stable LSD radix sort on the tile id; stability keeps the
 //                 primitive ids ascending inside each tile.
 //   k_tile_bounds CSR offsets from the boundaries of the sorted tile ids.
+// For few lists x primitives, the lists are built directly instead:
+//   k_tile_count  one workgroup per list: its primitives, counted per quarter
+//                 of the primitive range (one quarter per wave).
+//   k_tile_fill   one workgroup per list: its offset from the counts, then each
+//                 wave appends its quarter's primitives in id order (ballot
+//                 compaction), so the lists equal the sorted ones.
 #pragma once
 #include "dbmp_device.h"
 
@@ -444,6 +450,101 @@ static __global__ __launch_bounds__(256) void k_tile_bounds(const int* __restric
   const int kcur = (i < P) ? keys[i] : tiles;
   const int kprev = (i > 0) ? keys[i - 1] : -1;
   for (int t = kprev + 1; t <= kcur; ++t) tile_offsets[t] = (int)i;
+}
+
+// Direct list build.  A primitive n belongs to list t = (tx, ty) (lists of
+// 2^shift x 2^shift tiles, tilesX lists per row) iff its tile range rg[n]
+// covers it.  Wave w of a workgroup owns primitives [w*q, (w+1)*q), q =
+// ceil(N/4); eight range loads per lane are in flight at a time.
+constexpr int kDirectBatch = 8;
+
+__device__ __forceinline__ bool list_covers(int4 r, int tx, int ty, int shift) {
+  return ((r.x >> shift) <= tx) & (tx <= (r.y >> shift)) & ((r.z >> shift) <= ty) &
+         (ty <= (r.w >> shift));
+}
+
+static __global__ __launch_bounds__(256) void k_tile_count(const int4* __restrict__ rg, int N,
+                                                           int tilesX, int shift,
+                                                           int* __restrict__ counts4) {
+  const int t = blockIdx.x;
+  const int tx = t % tilesX, ty = t / tilesX;
+  const int w = threadIdx.x >> 6, l = lane_id();
+  const int q = (N + 3) / 4;
+  const int lo = w * q, hi = min(N, lo + q);
+  int c = 0;
+  for (int base = lo; base < hi; base += 64 * kDirectBatch) {
+    int4 r[kDirectBatch];
+#pragma unroll
+    for (int k = 0; k < kDirectBatch; ++k) {
+      const int n = base + k * 64 + l;
+      r[k] = n < hi ? rg[n] : make_int4(0, -1, 0, -1);
+    }
+#pragma unroll
+    for (int k = 0; k < kDirectBatch; ++k) c += list_covers(r[k], tx, ty, shift) ? 1 : 0;
+  }
+  c = wave_sum_i32(c);
+  if (l == 0) counts4[4 * t + w] = c;
+}
+
+// offs (tiles+1): list offsets; total: pair count.  Lists are written only
+// when the pairs fit cap (otherwise the raster kernels walk all primitives).
+static __global__ __launch_bounds__(256) void k_tile_fill(const int4* __restrict__ rg, int N,
+                                                          int tilesX, int tiles, int shift,
+                                                          const int* __restrict__ counts4,
+                                                          long long cap, int* __restrict__ offs,
+                                                          int* __restrict__ idx,
+                                                          long long* __restrict__ total) {
+  __shared__ long long sred[2][4];
+  const int t = blockIdx.x;
+  const int tx = t % tilesX, ty = t / tilesX;
+  const int w = threadIdx.x >> 6, l = lane_id();
+  // pairs of all lists before this one, and of all lists
+  long long before = 0, all = 0;
+  for (int i = threadIdx.x; i < 4 * tiles; i += 256) {
+    const long long c = counts4[i];
+    all += c;
+    before += (i < 4 * t) ? c : 0;
+  }
+  before = wave_sum_i64(before);
+  all = wave_sum_i64(all);
+  if (l == 0) {
+    sred[0][w] = before;
+    sred[1][w] = all;
+  }
+  __syncthreads();
+  before = sred[0][0] + sred[0][1] + sred[0][2] + sred[0][3];
+  all = sred[1][0] + sred[1][1] + sred[1][2] + sred[1][3];
+  // on overflow (all > cap) every offset is 0, as k_tile_bounds leaves them,
+  // and the raster kernels walk every primitive
+  const bool fits = all <= cap;
+  if (threadIdx.x == 0) {
+    offs[t] = fits ? (int)before : 0;
+    if (t == tiles - 1) {
+      offs[tiles] = fits ? (int)all : 0;
+      *total = all;
+    }
+  }
+  if (!fits) return;
+  long long pos = before;
+  for (int k = 0; k < w; ++k) pos += counts4[4 * t + k];
+  const int q = (N + 3) / 4;
+  const int lo = w * q, hi = min(N, lo + q);
+  const unsigned long long lt = (1ull << l) - 1ull;
+  for (int base = lo; base < hi; base += 64 * kDirectBatch) {
+    int4 r[kDirectBatch];
+#pragma unroll
+    for (int k = 0; k < kDirectBatch; ++k) {
+      const int n = base + k * 64 + l;
+      r[k] = n < hi ? rg[n] : make_int4(0, -1, 0, -1);
+    }
+#pragma unroll
+    for (int k = 0; k < kDirectBatch; ++k) {
+      const bool in = list_covers(r[k], tx, ty, shift);
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
+      if (in) idx[pos + __popcll(m & lt)] = base + k * 64 + l;
+      pos += __popcll(m);
+    }
+  }
 }
 
 }  // namespace dbmp

@@ -171,6 +171,16 @@ __device__ __forceinline__ float bilerp_grad(float v00, float v01, float v10, fl
 // Wave-level helpers (wave64) ---------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+__device__ __forceinline__ int wave_sum_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 __device__ __forceinline__ int wave_max_i32(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));

@@ -647,34 +647,98 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
                                      none, Cfin, Tfin, lastin, grad8);
 }
 
-// Heaviest-first order of nb blocks from their forward costs: 256 cost
+// Heaviest-first order of nb blocks from their forward costs: 64 cost
 // buckets (bucket 0 the heaviest), blocks within a bucket in any order.
-// One 256-thread workgroup.  Launching the heavy blocks first keeps the last
-// workgroups of a launch short, so the machine stays full until the end.
+// One 256-thread workgroup.  The costs are staged in LDS with every load in
+// flight; lanes of a wave that fall in the same bucket are found with ballots
+// (one per bucket bit) and counted with a single LDS atomic, so a crowded
+// bucket costs one atomic per wave, not one per lane.  Launching the heavy
+// blocks first keeps the last workgroups of a launch short, so the machine
+// stays full until the end.  (Ordering whole super-tile groups instead was
+// measured: the order kernel halves, the raster kernels lose more.)
 constexpr int kOrderThreads = 256;
-static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int* __restrict__ cost,
+constexpr int kOrderBits = 6;
+constexpr int kOrderLds = 16384;   // costs staged in LDS up to this many blocks
+static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int* __restrict__ cost_g,
                                                                     int nb, int* __restrict__ order) {
-  __shared__ int cnt[256];
+  __shared__ int cnt[1 << kOrderBits];
   __shared__ int wmax[4];
-  __shared__ long long sh[4];
-  cnt[threadIdx.x] = 0;
+  __shared__ int scost[kOrderLds];
+  if (threadIdx.x < (1 << kOrderBits)) cnt[threadIdx.x] = 0;
+  const bool staged = nb <= kOrderLds;
+  const int* cost = staged ? scost : cost_g;
   int m = 1;
-  for (int i = threadIdx.x; i < nb; i += kOrderThreads) m = max(m, cost[i]);
+  if (staged) {
+    const int n4 = nb >> 2;
+    const int4* c4 = reinterpret_cast<const int4*>(cost_g);
+#pragma unroll 4
+    for (int i = threadIdx.x; i < n4; i += kOrderThreads) {
+      const int4 c = c4[i];
+      scost[4 * i] = c.x; scost[4 * i + 1] = c.y; scost[4 * i + 2] = c.z; scost[4 * i + 3] = c.w;
+      m = max(max(m, max(c.x, c.y)), max(c.z, c.w));
+    }
+    for (int i = 4 * n4 + threadIdx.x; i < nb; i += kOrderThreads) {
+      scost[i] = cost_g[i];
+      m = max(m, cost_g[i]);
+    }
+  } else {
+    for (int i = threadIdx.x; i < nb; i += kOrderThreads) m = max(m, cost_g[i]);
+  }
   m = wave_max_i32(m);
   if (lane_id() == 0) wmax[threadIdx.x >> 6] = m;
   __syncthreads();
-  const long long mx = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-  auto bucket = [&](int c) {
-    const long long q = ((long long)max(c, 0) * 255 + mx - 1) / mx;   // 0..255, ascending cost
-    return 255 - (int)min(q, 255LL);
+  const int mx = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+  constexpr int kTop = (1 << kOrderBits) - 1;
+  const float qs = (float)kTop / (float)mx;
+  auto bucket = [&](int c) {   // 0 for the heaviest costs; any monotone map serves
+    return kTop - min((int)((float)max(c, 0) * qs), kTop);
   };
-  for (int i = threadIdx.x; i < nb; i += kOrderThreads) atomicAdd(&cnt[bucket(cost[i])], 1);
+  const int l = lane_id();
+  const unsigned long long lt = (1ull << l) - 1ull;
+  // lanes of this wave with the same bucket as this lane (among valid lanes)
+  auto peers_of = [&](bool ok, int b) {
+    unsigned long long p = __builtin_amdgcn_ballot_w64(ok);
+#pragma unroll
+    for (int k = 0; k < kOrderBits; ++k) {
+      const bool bit = (b >> k) & 1;
+      const unsigned long long mk = __builtin_amdgcn_ballot_w64(bit);
+      p &= bit ? mk : ~mk;
+    }
+    return p;
+  };
+  const int rounds = (nb + kOrderThreads - 1) / kOrderThreads;
+#pragma unroll 4
+  for (int r = 0; r < rounds; ++r) {
+    const int i = r * kOrderThreads + threadIdx.x;
+    const bool ok = i < nb;
+    const int b = ok ? bucket(cost[i]) : 0;
+    const unsigned long long p = peers_of(ok, b);
+    if (ok && (p & lt) == 0ull) atomicAdd(&cnt[b], __popcll(p));
+  }
   __syncthreads();
-  long long tot;
-  const int ex = (int)block256_excl_scan(cnt[threadIdx.x], tot, sh);
-  cnt[threadIdx.x] = ex;
+  if (threadIdx.x < 64) {   // exclusive scan of the 64 bucket counts in wave 0
+    const int c = cnt[threadIdx.x];
+    int inc = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(inc, o, 64);
+      if (l >= o) inc += t;
+    }
+    cnt[threadIdx.x] = inc - c;
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += kOrderThreads) order[atomicAdd(&cnt[bucket(cost[i])], 1)] = i;
+#pragma unroll 4
+  for (int r = 0; r < rounds; ++r) {
+    const int i = r * kOrderThreads + threadIdx.x;
+    const bool ok = i < nb;
+    const int b = ok ? bucket(cost[i]) : 0;
+    const unsigned long long p = peers_of(ok, b);
+    const int leader = ok ? __builtin_ctzll(p) : 0;
+    int base = 0;
+    if (ok && l == leader) base = atomicAdd(&cnt[b], __popcll(p));
+    base = __shfl(base, leader, 64);
+    if (ok) order[base + __popcll(p & lt)] = i;
+  }
 }
 
 }  // namespace dbmp
