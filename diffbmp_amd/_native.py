@@ -36,7 +36,9 @@ class AdamGroup(ctypes.Structure):
                 ("exp_avg_sq", ctypes.c_void_p), ("grad", ctypes.c_void_p),
                 ("store", ctypes.c_void_p), ("numel", ctypes.c_int64), ("lr", ctypes.c_float),
                 ("stride", ctypes.c_int32), ("clamp", ctypes.c_int32),
-                ("lo", ctypes.c_float * 2), ("hi", ctypes.c_float * 2)]
+                ("lo", ctypes.c_float * 2), ("hi", ctypes.c_float * 2),
+                ("grad_row", ctypes.c_int32), ("grad_comps", ctypes.c_int32),
+                ("zero_grad", ctypes.c_int32)]
 
 
 class Adam(ctypes.Structure):
@@ -145,6 +147,12 @@ EXPORTS = {
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_backward_mse_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims),
+                                              ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
+                                              ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p]),
     "dbmp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Adam), ctypes.c_void_p]),
     "dbmp_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dbmp_timing_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
@@ -426,6 +434,18 @@ class Handle:
                                         _ptr(loss_accum), _ptr(out_color), _ptr(out_T), _ptr(out_last),
                                         _ptr(g_means), _ptr(g_r), _ptr(g_th), _ptr(g_v), _ptr(g_c),
                                         _stream(target.device)))
+
+    def backward_mse_rows(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T, target, out_color,
+                          out_T, out_last, grad_rows, loss_accum=None, bg=None,
+                          bg_value: float = 1.0) -> None:
+        """backward_mse that adds the gradients into ``grad_rows`` ((N,8) fp32:
+        x, y, r, theta, opacity logit, 3 colour logits)."""
+        L = lists.c()
+        check(self._L.dbmp_backward_mse_rows(self._h, ctypes.byref(prims), ctypes.byref(bm),
+                                             ctypes.byref(L), float(eps_T), _ptr(target), _ptr(bg),
+                                             float(bg_value), _ptr(loss_accum), _ptr(out_color),
+                                             _ptr(out_T), _ptr(out_last), _ptr(grad_rows),
+                                             _stream(target.device)))
 
     def loss_mse(self, color, alpha, target, grad_color, grad_alpha, loss_accum=None, bg=None,
                  bg_value: float = 1.0) -> None:

@@ -578,28 +578,28 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   return 0;
 }
 
-int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
-                      const dbmp_tile_lists* lists, float eps_T, const float* target,
-                      const float* background, float background_value, float* loss_accum,
-                      const float* out_color, const float* out_T_final, const int32_t* out_last,
+// Shared body of dbmp_backward_mse (rows = null: internal rows, zeroed first,
+// then unpacked) and dbmp_backward_mse_rows (accumulate into the caller's rows).
+static int backward_mse_impl(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                      const dbmp_tile_lists* lists, const float* target, const float* background,
+                      float background_value, float* loss_accum, const float* out_color,
+                      const float* out_T_final, const int32_t* out_last, float* rows,
                       float* grad_means2D, float* grad_radii, float* grad_rotations,
-                      float* grad_opacities, float* grad_colors, void* stream) {
-  (void)eps_T;
+                      float* grad_opacities, float* grad_colors, hipStream_t st) {
   if (!h) return fail(DBMP_E_ARG, "handle is NULL");
   int rc = check_prims(prims);
   if (rc) return rc;
   if ((rc = check_bitmaps(bm))) return rc;
   if ((rc = check_lists(h, lists))) return rc;
   if (!target || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
-  if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
+  if (!rows && (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors))
     return fail(DBMP_E_ARG, "NULL gradient output");
-  hipStream_t st = (hipStream_t)stream;
   const int N = prims->num_primitives;
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
   timing_begin(h, h->ev_bwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
-  if (N > 0) {
+  if (N > 0 && !rows) {
     hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
     LAUNCH_CHECK("k_fill_f32");
   }
@@ -613,15 +613,40 @@ int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bi
   L.loss = loss_accum;
   const bool blend = A.cmap != nullptr;
   DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd_mse, dim3(A.tiles * A.subs), dim3(kThreads), 0, st,
-                    A, L, out_color, out_T_final, out_last, h->grad8);
+                    A, L, out_color, out_T_final, out_last, rows ? rows : h->grad8);
   LAUNCH_CHECK("k_raster_bwd_mse");
-  if (N > 0) {
+  if (N > 0 && !rows) {
     hipLaunchKernelGGL(k_unpack_grads, dim3(ceil_div(N, 256)), dim3(256), 0, st, h->grad8, N,
                        grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);
     LAUNCH_CHECK("k_unpack_grads");
   }
   timing_end(h, h->ev_bwd, st);
   return 0;
+}
+
+int dbmp_backward_mse(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                      const dbmp_tile_lists* lists, float eps_T, const float* target,
+                      const float* background, float background_value, float* loss_accum,
+                      const float* out_color, const float* out_T_final, const int32_t* out_last,
+                      float* grad_means2D, float* grad_radii, float* grad_rotations,
+                      float* grad_opacities, float* grad_colors, void* stream) {
+  (void)eps_T;
+  return backward_mse_impl(h, prims, bm, lists, target, background, background_value, loss_accum,
+                           out_color, out_T_final, out_last, nullptr, grad_means2D, grad_radii,
+                           grad_rotations, grad_opacities, grad_colors, (hipStream_t)stream);
+}
+
+int dbmp_backward_mse_rows(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                           const dbmp_tile_lists* lists, float eps_T, const float* target,
+                           const float* background, float background_value, float* loss_accum,
+                           const float* out_color, const float* out_T_final,
+                           const int32_t* out_last, float* grad_rows, void* stream) {
+  (void)eps_T;
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  if (!grad_rows && prims && prims->num_primitives > 0) return fail(DBMP_E_ARG, "grad_rows is NULL");
+  return backward_mse_impl(h, prims, bm, lists, target, background, background_value, loss_accum,
+                           out_color, out_T_final, out_last, grad_rows ? grad_rows : h->grad8,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream);
 }
 
 int dbmp_loss_mse(dbmp_rasterizer* h, const float* color, const float* alpha, const float* bg,
@@ -656,6 +681,10 @@ int dbmp_adam_step(dbmp_rasterizer* h, const dbmp_adam* cfg, void* stream) {
         return fail(DBMP_E_ARG, "Adam group %d has a NULL pointer", i);
       g.p = c.param; g.m = c.exp_avg; g.v = c.exp_avg_sq; g.g = c.grad; g.store = c.store;
       g.n = c.numel; g.stride = c.stride; g.clamp = c.clamp;
+      if (c.grad_row < 0 || (c.grad_row > 0 && (c.grad_comps < 1 || c.grad_comps > c.grad_row)))
+        return fail(DBMP_E_ARG, "Adam group %d: bad grad_row %d / grad_comps %d", i, c.grad_row,
+                    c.grad_comps);
+      g.grow = c.grad_row; g.gcomp = c.grad_row > 0 ? c.grad_comps : 1; g.zero = c.zero_grad != 0;
       g.lo[0] = c.lo[0]; g.lo[1] = c.lo[1]; g.hi[0] = c.hi[0]; g.hi[1] = c.hi[1];
       A.step_scale[i] = (float)((double)c.lr / bc1);
       nmax = std::max(nmax, (long long)c.numel);

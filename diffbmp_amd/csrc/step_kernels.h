@@ -48,12 +48,14 @@ struct AdamGroup {
   float* p;          // fp32 master parameter
   float* m;          // first moment
   float* v;          // second moment
-  const float* g;    // gradient
+  float* g;          // gradient (element i at g[i], or at a row column, below)
   void* store;       // optional reduced-precision copy for the rasterizer (or null)
   long long n;       // elements
   int stride;        // 2 for means2D (component c = i % 2 uses lo[c], hi[c]), else 1
   int clamp;         // apply [lo, hi] after the update
   float lo[2], hi[2];
+  int grow, gcomp;   // grow > 0: element i's gradient at g[(i / gcomp) * grow + i % gcomp]
+  int zero;          // write 0 over each gradient value read
 };
 
 struct AdamArgs {
@@ -73,7 +75,9 @@ static __global__ __launch_bounds__(256) void k_adam(AdamArgs A) {
   const float ss = A.step_scale[blockIdx.y];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G.n;
        i += (long long)gridDim.x * blockDim.x) {
-    const float g = G.g[i];
+    const long long gi = G.grow > 0 ? (i / G.gcomp) * G.grow + i % G.gcomp : i;
+    const float g = G.g[gi];
+    if (G.zero) G.g[gi] = 0.f;
     const float m = fmaf(A.beta1, G.m[i], (1.f - A.beta1) * g);
     const float v = fmaf(A.beta2, G.v[i], (1.f - A.beta2) * (g * g));
     G.m[i] = m;

@@ -22,6 +22,9 @@ from ._native import Adam, AdamGroup, Handle
 __all__ = ["FitJob", "GROUPS"]
 
 GROUPS = ("means2D", "radii", "rotations", "opacities", "colors")
+# first column and width of each group in the (N,8) gradient rows
+_ROW_COLUMN = {"means2D": (0, 2), "radii": (2, 1), "rotations": (3, 1), "opacities": (4, 1),
+               "colors": (5, 3)}
 
 
 class FitJob:
@@ -57,7 +60,9 @@ class FitJob:
                        for k, t in zip(GROUPS, (means2D, radii, rotations, opacities, colors))}
         self.exp_avg = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.exp_avg_sq = {k: torch.zeros_like(v) for k, v in self.params.items()}
-        self.grads = {k: torch.zeros_like(v) for k, v in self.params.items()}
+        # gradient rows (N,8): x, y, r, theta, opacity logit, 3 colour logits.
+        # The backward adds into them and the Adam step zeroes what it reads.
+        self.grad_rows = torch.zeros((self.N, 8), dtype=f32, device=dev)
         if store_dtype == f32:
             self.store = self.params
         else:
@@ -88,7 +93,9 @@ class FitJob:
             g = self.adam.groups[i]
             p = self.params[k]
             g.param, g.exp_avg, g.exp_avg_sq = p.data_ptr(), self.exp_avg[k].data_ptr(), self.exp_avg_sq[k].data_ptr()
-            g.grad = self.grads[k].data_ptr()
+            col, comps = _ROW_COLUMN[k]
+            g.grad = self.grad_rows.data_ptr() + 4 * col
+            g.grad_row, g.grad_comps, g.zero_grad = 8, comps, 1
             g.store = self.store[k].data_ptr() if store_dtype != f32 else None
             g.numel = p.numel()
             g.lr = float(lrs[k])
@@ -116,11 +123,11 @@ class FitJob:
         lists = h.bin_lazy(self._prims(False), self.ext, tuple(self.templates.shape), h.bin_factor)
         self.lists = lists
         h.forward(self._prims(True), self._bm, lists, self.eps_T, self.color, self.alpha, self.T, self.last)
-        g = self.grads
-        # background composite + MSE + its gradient fused into the backward pass
-        h.backward_mse(self._prims(True), self._bm, lists, self.eps_T, self.target, self.color, self.T,
-                       self.last, g["means2D"], g["radii"], g["rotations"], g["opacities"], g["colors"],
-                       loss_accum=self.losses[self.t:self.t + 1], bg=self.bg, bg_value=1.0)
+        # background composite + MSE + its gradient fused into the backward
+        # pass, added into the gradient rows the Adam step then reads and clears
+        h.backward_mse_rows(self._prims(True), self._bm, lists, self.eps_T, self.target, self.color,
+                            self.T, self.last, self.grad_rows,
+                            loss_accum=self.losses[self.t:self.t + 1], bg=self.bg, bg_value=1.0)
         self.t += 1
         self.adam.step = self.t
         h.adam_step(self.adam, self.device)

@@ -140,6 +140,17 @@ int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext
                    int32_t* tile_indices, int64_t capacity, int64_t* d_pair_count,
                    int32_t* ranges_out, void* stream);
 
+/* dbmp_backward_mse that ADDS the gradients into grad_rows, (N,8) fp32 rows
+ * (x, y, r, theta, opacity logit, 3 colour logits), and skips the per-array
+ * unpacking.  With dbmp_adam_step reading the rows (grad_row 8, zero_grad 1)
+ * the rows are zero again for the next step.  No reference counterpart: it is
+ * the device-resident step's form of rasterize_tiles_backward_class. */
+int dbmp_backward_mse_rows(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                           const dbmp_tile_lists* lists, float eps_T, const float* target,
+                           const float* background, float background_value, float* loss_accum,
+                           const float* out_color, const float* out_T_final,
+                           const int32_t* out_last, float* grad_rows, void* stream);
+
 /* Forward; replaces _C.rasterize_tiles_class / _C_fp16.rasterize_tiles_class_fp16
  * (ext.cpp:11-28 -> tile_rasterize.cu:297-498).  Outputs (fp32):
  *   out_color (H,W,3) premultiplied over black, out_alpha (H,W),
@@ -180,13 +191,18 @@ typedef struct {
   float* param;
   float* exp_avg;
   float* exp_avg_sq;
-  const float* grad;
+  float* grad;
   void* store;          /* optional copy written in store_dtype (or NULL) */
   int64_t numel;
   float lr;
   int32_t stride;       /* 2 for means2D: element i uses lo[i%2], hi[i%2] */
   int32_t clamp;        /* clamp to [lo, hi] after the update */
   float lo[2], hi[2];
+  int32_t grad_row;     /* 0: element i's gradient is grad[i]; > 0: it is
+                           grad[(i / grad_comps) * grad_row + i % grad_comps]
+                           (e.g. a column of dbmp_backward_mse_rows' rows) */
+  int32_t grad_comps;
+  int32_t zero_grad;    /* 1: write 0 over each gradient value read */
 } dbmp_adam_group;
 
 typedef struct {

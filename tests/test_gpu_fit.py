@@ -62,6 +62,59 @@ def test_fused_backward_mse_matches_two_kernels(dev, with_bg):
         assert err <= 1e-6, (k, err)
 
 
+def test_backward_rows_accumulate_and_adam_clears_them(dev):
+    """dbmp_backward_mse_rows adds the (N,8) gradient rows that
+    dbmp_backward_mse unpacks; an Adam step with zero_grad reads the rows as
+    the unpacked arrays and leaves them zero."""
+    from helpers import GpuRun
+    from diffbmp_amd import _native
+    sc = S.make_scene(0, with_target=False)
+    run = GpuRun(sc, dev, eps_T=1e-6)
+    run.forward()
+    target = torch.rand((sc.H, sc.W, 3), device=dev, generator=torch.Generator(dev).manual_seed(8))
+    N = sc.N
+    lists = _native.Lists(run.offs, run.idx)
+    out = [torch.empty(sh, device=dev) for sh in ((N, 2), (N,), (N,), (N,), (N, 3))]
+    run.h.backward_mse(run.prims, run.bm, lists, run.eps, target, run.color, run.T, run.last, *out)
+    ref = torch.cat([out[0], out[1][:, None], out[2][:, None], out[3][:, None], out[4]], 1)
+    rows = torch.zeros((N, 8), device=dev)
+    run.h.backward_mse_rows(run.prims, run.bm, lists, run.eps, target, run.color, run.T, run.last, rows)
+    torch.cuda.synchronize()
+    assert torch.linalg.norm(rows - ref) <= 1e-6 * torch.linalg.norm(ref)
+    run.h.backward_mse_rows(run.prims, run.bm, lists, run.eps, target, run.color, run.T, run.last, rows)
+    torch.cuda.synchronize()
+    assert torch.linalg.norm(rows - 2 * ref) <= 1e-6 * torch.linalg.norm(ref)
+    # Adam over the row columns == Adam over the unpacked arrays; rows cleared
+    cols = {"means2D": (0, 2), "radii": (2, 1), "rotations": (3, 1), "opacities": (4, 1), "colors": (5, 3)}
+    params = [torch.rand(o.shape, device=dev) + 1.0 for o in out]
+    results = []
+    for use_rows in (False, True):
+        adam = _native.Adam()
+        adam.num_groups, adam.beta1, adam.beta2, adam.eps, adam.step = 5, 0.9, 0.999, 1e-8, 1
+        adam.store_dtype = _native.DBMP_F32
+        keep, ps = [], []
+        for i, (k, o) in enumerate(zip(cols, out)):
+            p = params[i].clone()
+            m, v = torch.zeros_like(p), torch.zeros_like(p)
+            keep += [m, v]
+            ps.append(p)
+            g = adam.groups[i]
+            g.param, g.exp_avg, g.exp_avg_sq = p.data_ptr(), m.data_ptr(), v.data_ptr()
+            g.numel, g.lr, g.stride, g.clamp = p.numel(), 0.01, 2 if k == "means2D" else 1, 0
+            if use_rows:
+                g.grad = rows.data_ptr() + 4 * cols[k][0]
+                g.grad_row, g.grad_comps, g.zero_grad = 8, cols[k][1], 1
+            else:
+                keep.append((2 * o).contiguous())
+                g.grad = keep[-1].data_ptr()
+        run.h.adam_step(adam, dev)
+        torch.cuda.synchronize()
+        results.append([q.clone() for q in ps])
+    for a, b in zip(*results):
+        assert torch.allclose(a, b, rtol=1e-6, atol=1e-7)
+    assert rows.abs().max().item() == 0.0
+
+
 def test_fitjob_tracks_autograd_adam(dev):
     sc = S.make_scene(0, with_target=False)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
