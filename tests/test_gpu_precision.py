@@ -7,8 +7,8 @@ For the 1024^2 / 8192-primitive workload, on a random sample of tiles:
     storage quantisation (fp16 positions move in 0.5 px steps at 512-1024 px,
     bf16 in 4 px steps; the reference's fp16 path casts means2D to half too,
     cuda_tile_rasterizer/__init__.py:348).  Reported, bounded loosely.
-The error table is printed and written to gpurun_out/precision_errors.json
-when that folder exists.
+The error table is printed, and also merged into the JSON file named by the
+environment variable DBMP_PRECISION_TABLE when it is set.
 """
 import json
 import os
@@ -61,8 +61,8 @@ def test_precision_error_table(dev, store):
         ref_img = np.concatenate([rc, ra[..., None]], -1)
         table[name] = _errs(img, ref_img, g, ref_g, pm)
     print(store, json.dumps(table, indent=1))
-    if os.path.isdir("gpurun_out"):
-        path = "gpurun_out/precision_errors.json"
+    path = os.environ.get("DBMP_PRECISION_TABLE")
+    if path:
         allt = json.load(open(path)) if os.path.exists(path) else {}
         allt[store] = table
         json.dump(allt, open(path, "w"), indent=1)
