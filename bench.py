@@ -391,7 +391,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the rasterizer "
-              "path has no multi-GPU mode in this build.", file=sys.stderr)
+              "path has no multi-GPU mode.", file=sys.stderr)
         sys.exit(2)
     import torch
     from diffbmp_amd import scenes as S
