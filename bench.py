@@ -124,7 +124,14 @@ def run_fused(args, sc, dev, sdt):
                  max_steps=args.warmup + 2 * args.steps + 16)
     if args.eps_t is not None:
         job.eps_T = args.eps_t
-    return job, job.step, job.handle
+
+    def fwd_bwd():
+        job.fwd_bwd()
+
+    def reset():
+        job.grad_rows.zero_()
+
+    return job, job.step, job.handle, fwd_bwd, reset
 
 
 def run_autograd(args, sc, dev, sdt):
@@ -145,23 +152,26 @@ def run_autograd(args, sc, dev, sdt):
     rmax = float(min(H, W) // 4)
     templates, sel, target = pr["templates"], pr["sel"], pr["target"]
 
-    def step():
+    def fwd_bwd():
         opt.zero_grad(set_to_none=True)
         color, alpha = rast(*leaves, empty, templates, sel, 0.0, lr_conf, None)
         loss = F.mse_loss(color.float() + (1.0 - alpha.float()).unsqueeze(-1) * bg, target)
         loss.backward()
+
+    def step():
+        fwd_bwd()
         opt.step()
         with torch.no_grad():
             leaves[0][:, 0].clamp_(0.0, float(W - 1))
             leaves[0][:, 1].clamp_(0.0, float(H - 1))
             leaves[1].clamp_(2.0, rmax)
 
-    return rast, step, rast.handle
+    return rast, step, rast.handle, fwd_bwd, (lambda: None)
 
 
 def time_mode(args, runner, sc, dev, sdt):
     import torch
-    obj, step, handle = runner(args, sc, dev, sdt)
+    obj, step, handle, fwd_bwd, reset = runner(args, sc, dev, sdt)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -184,8 +194,17 @@ def time_mode(args, runner, sc, dev, sdt):
         e1.record(stream)
         e1.synchronize()
         lat.append(e0.elapsed_time(e1))
+    # the hot path alone (binning + forward + backward, with the loss that
+    # feeds it), hipEvents on the stream around K passes
+    e0.record(stream)
+    for _ in range(args.steps):
+        fwd_bwd()
+    e1.record(stream)
+    e1.synchronize()
+    fb_s = e0.elapsed_time(e1) * 1e-3 / args.steps
+    reset()
     return dict(obj=obj, handle=handle, total=t1 - t0, lat=np.array(lat),
-                fwd_s=f_ms / max(f_n, 1) * 1e-3, bwd_s=b_ms / max(b_n, 1) * 1e-3)
+                fwd_s=f_ms / max(f_n, 1) * 1e-3, bwd_s=b_ms / max(b_n, 1) * 1e-3, fwd_bwd_s=fb_s)
 
 
 def psnr_compare(args, sc, dev, sdt):
@@ -462,6 +481,11 @@ def main():
         "raster_fwd_ms": round(res["fwd_s"] * 1e3, 4),
         "raster_bwd_ms": round(res["bwd_s"] * 1e3, 4),
         f"{other}_iters_per_s": round(args.steps / res_other["total"], 3),
+        # binning + forward + backward alone (the step above adds the Adam
+        # update and clamp)
+        "fwd_bwd_iters_per_s": round(1.0 / res["fwd_bwd_s"], 3),
+        f"{other}_fwd_bwd_iters_per_s": round(1.0 / res_other["fwd_bwd_s"], 3),
+        "samples_per_s_fwd": float(f"{walked * H * W / res['fwd_s']:.4g}"),
         "pairs": P,
         "list_len_mean": round(list_mean, 2),
         "entries_walked_per_pixel": round(walked, 2),

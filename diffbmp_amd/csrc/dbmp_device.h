@@ -16,13 +16,16 @@ enum DType : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
 
 // Per-primitive constants, computed once per call by the prep kernel and
 // read by the raster kernels through uniform (scalar) loads.  64 bytes.
+// The raster loops read the first three 16-byte quarters (one LDS load each);
+// the fourth holds what only the binning and the gradient reduction use.
 struct alignas(16) PrimRec {
   float mx, my, hw, hh;      // centre; push-box half extents (tile_forward.cu:195-210)
   float c, s, rinv, abase;   // cos, sin (fp32, correctly rounded), 1/r, alpha_ub * sigmoid(v)
-  float cr, cg, cb, sop;     // sigmoid(colour logits), sigmoid(opacity logit)
+  float cr, cg, cb;          // sigmoid(colour logits)
+  int tbase;                 // tmpl * Ht * Wt: first texel quad of the template
   int tmpl;                  // template index, -1 => never pushed
   float r;                   // radius
-  int tbase;                 // tmpl * Ht * Wt: first texel quad of the template
+  float sop;                 // sigmoid(opacity logit)
   float dav;                 // alpha_ub * s (1 - s): d alpha / d(opacity logit) per unit mask
 };
 static_assert(sizeof(PrimRec) == 64, "PrimRec is 64 bytes");

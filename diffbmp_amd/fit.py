@@ -132,6 +132,20 @@ class FitJob:
         self.adam.step = self.t
         h.adam_step(self.adam, self.device)
 
+    def fwd_bwd(self) -> None:
+        """Binning, forward and backward of the current parameters, without the
+        Adam update: the hot path alone.  The gradients are added into
+        ``grad_rows`` (call ``grad_rows.zero_()`` before the next ``step``) and
+        the loss into a scratch value, not the loss history."""
+        h = self.handle
+        if not hasattr(self, "_loss_scratch"):
+            self._loss_scratch = torch.zeros(1, dtype=torch.float32, device=self.device)
+        lists = h.bin_lazy(self._prims(False), self.ext, tuple(self.templates.shape), h.bin_factor)
+        h.forward(self._prims(True), self._bm, lists, self.eps_T, self.color, self.alpha, self.T, self.last)
+        h.backward_mse_rows(self._prims(True), self._bm, lists, self.eps_T, self.target, self.color,
+                            self.T, self.last, self.grad_rows, loss_accum=self._loss_scratch, bg=self.bg,
+                            bg_value=1.0)
+
     def render(self) -> torch.Tensor:
         """Composite image (H,W,3) of the current parameters over the background."""
         h = self.handle
