@@ -480,6 +480,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     ga = (okpix && gAin) ? gAin[pix] : 0.f;
   }
   float T = 1.f, ca0 = 0.f, ca1 = 0.f, ca2 = 0.f;
+  const float gcf = fmaf(gc2, cf2, fmaf(gc1, cf1, gc0 * cf0));   // gc . C_final
   const float hyx = A.hsy / A.hsx;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
@@ -521,33 +522,35 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const float a = S.pushed ? p.abase * S.m : 0.f;
       const float Tk = T;
       const float wgt = Tk * a;
-      // suffix colour S_k = C_final - sum_{m<=k} T_m a_m c_m, the prefix
-      // recomputed with the forward's exact operations (S = 0 at the end)
+      // colour prefix C_k = sum_{m<=k} T_m a_m c_m with the forward's exact
+      // operations, so the suffix S_k = C_final - C_k is 0 at the end; the
+      // loss only needs its projection on gc: gc.S = gc.C_final - gc.C_k,
+      // both dot products formed in the same order (equal when C_k = C_final)
       ca0 = fmaf(wgt, col[0], ca0);
       ca1 = fmaf(wgt, col[1], ca1);
       ca2 = fmaf(wgt, col[2], ca2);
-      const float S0 = cf0 - ca0, S1 = cf1 - ca1, S2 = cf2 - ca2;
+      const float gS = gcf - fmaf(gc2, ca2, fmaf(gc1, ca1, gc0 * ca0));
+      const float gcol = fmaf(gc2, col[2], fmaf(gc1, col[1], gc0 * col[0]));
       const float om = 1.f - a;
       // max(om, 1e-6) as one med3 (om <= 1 - 0 for a >= 0; FLT_MAX bounds it otherwise)
       const float inv1m = __builtin_amdgcn_rcpf(__builtin_amdgcn_fmed3f(om, 1e-6f, 3.402823466e38f));
       const float fk = fmaxf(om, 0.f);
       const float Tn = Tk * fk;
-      // suffix transmittance B_k = prod_{m>k} (1 - a_m) = T_final / T_{k+1},
-      // 1 once T is 0; for an exactly opaque layer (fk = 0) it is the
+      // T_k B_k, with the suffix transmittance B_k = prod_{m>k} (1 - a_m) =
+      // T_final / T_{k+1}: T_k B_k = T_final / (1 - a_k) while 1 - a_k >= 1e-6
+      // (it is 0 = T_final once T is 0).  Nearer-opaque layers, rare, take
+      // the general form: for an exactly opaque layer B_k is the
       // nonzero-factor product after it, tnzf / T_k, if it is the only opaque
-      // layer, else 0 (selects, one reciprocal per lane)
-      const bool tpos = Tn > 0.f;
-      float B;
-      if (wave_any(fk == 0.f)) {
+      // layer, else 0
+      float TkB;
+      if (wave_any(om < 1e-6f)) {
+        const bool tpos = Tn > 0.f;
         const float rB = fminf((tpos ? tf : tnzf) * __builtin_amdgcn_rcpf(tpos ? Tn : Tk), 1.f);
-        B = tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f);
+        TkB = Tk * (tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f));
       } else {
-        B = tpos ? fminf(tf * __builtin_amdgcn_rcpf(Tn), 1.f) : 1.f;
+        TkB = tf * inv1m;
       }
-      float dLda = ga * (Tk * B);
-      dLda = fmaf(gc0, fmaf(Tk, col[0], -S0 * inv1m), dLda);
-      dLda = fmaf(gc1, fmaf(Tk, col[1], -S1 * inv1m), dLda);
-      dLda = fmaf(gc2, fmaf(Tk, col[2], -S2 * inv1m), dLda);
+      const float dLda = fmaf(ga, TkB, fmaf(Tk, gcol, -gS * inv1m));
       // Per-sample parts of the 8 gradients.  Factors that are constant per
       // primitive (-1/r, dalpha/dv, sigma'(c), the blend weight) are applied
       // once per (workgroup, entry) when the wave partials are summed below.
@@ -558,11 +561,11 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       acc[5] = gc0 * wgt;
       acc[6] = gc1 * wgt;
       acc[7] = gc2 * wgt;
-      // opacity logit (:311-312): alpha_ub * mask * s (1 - s)
-      const float me = S.pushed ? S.m : 0.f;
-      acc[4] = dLda * me;
-      // mask path (:314-343)
-      const float dLdm = S.pushed ? dLda * p.abase : 0.f;
+      // opacity logit (:311-312): alpha_ub * mask * s (1 - s); mask path
+      // (:314-343); both zero on lanes not pushed
+      const float dLdap = S.pushed ? dLda : 0.f;
+      acc[4] = dLdap * S.m;
+      const float dLdm = dLdap * p.abase;
       // dL/du and dL/dv divided by hsx = dtx/du (hsx is applied with the
       // per-primitive factors); hyx = hsy / hsx
       float dLdu = dLdm * S.gxt;
