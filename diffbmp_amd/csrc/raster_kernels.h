@@ -652,20 +652,21 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
 
 // Heaviest-first order of nb blocks from their forward costs: 64 cost
 // buckets (bucket 0 the heaviest), blocks within a bucket in any order.
-// One 256-thread workgroup.  The costs are staged in LDS with every load in
+// One 1024-thread workgroup (16 waves: the kernel is latency bound, so
+// more waves mean fewer dependent rounds).  The costs are staged in LDS with every load in
 // flight; lanes of a wave that fall in the same bucket are found with ballots
 // (one per bucket bit) and counted with a single LDS atomic, so a crowded
 // bucket costs one atomic per wave, not one per lane.  Launching the heavy
 // blocks first keeps the last workgroups of a launch short, so the machine
 // stays full until the end.  (Ordering whole super-tile groups instead was
 // measured: the order kernel halves, the raster kernels lose more.)
-constexpr int kOrderThreads = 256;
+constexpr int kOrderThreads = 1024;
 constexpr int kOrderBits = 6;
 constexpr int kOrderLds = 16384;   // costs staged in LDS up to this many blocks
 static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int* __restrict__ cost_g,
                                                                     int nb, int* __restrict__ order) {
   __shared__ int cnt[1 << kOrderBits];
-  __shared__ int wmax[4];
+  __shared__ int wmax[kOrderThreads / 64];
   __shared__ int scost[kOrderLds];
   if (threadIdx.x < (1 << kOrderBits)) cnt[threadIdx.x] = 0;
   const bool staged = nb <= kOrderLds;
@@ -690,7 +691,9 @@ static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int*
   m = wave_max_i32(m);
   if (lane_id() == 0) wmax[threadIdx.x >> 6] = m;
   __syncthreads();
-  const int mx = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+  int mx = wmax[0];
+#pragma unroll
+  for (int w = 1; w < kOrderThreads / 64; ++w) mx = max(mx, wmax[w]);
   constexpr int kTop = (1 << kOrderBits) - 1;
   const float qs = (float)kTop / (float)mx;
   auto bucket = [&](int c) {   // 0 for the heaviest costs; any monotone map serves
