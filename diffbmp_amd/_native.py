@@ -230,6 +230,8 @@ class Handle:
     """One native rasterizer (``dbmp_init``): canvas, tile size, alpha bound,
     and device scratch.  Calls are ordered on the current torch stream."""
 
+    READBACK_EVERY = 8   # bin_lazy: calls per asynchronous pair-count read-back
+
     def __init__(self, image_height: int, image_width: int, tile_size: int = 16,
                  sigma: float = 0.0, alpha_upper_bound: float = 1.0,
                  max_prims_per_pixel: int = 151, num_primitives: int | None = None,
@@ -252,6 +254,7 @@ class Handle:
         # (same per-tile work, a sixteenth of the pairs to sort)
         self.bin_factor = 4 if self.ts == 16 else 1
         self._pending = None          # (event, pinned count) of the last async binning
+        self._lazy_calls = 0
         self._tmpl_cache = None       # (key, converted templates, extents, version)
         self._tmpl_gen = 0
 
@@ -380,12 +383,14 @@ class Handle:
 
     def bin_lazy(self, prims: Prims, ext: torch.Tensor, tmpl_shape, factor: int = 1) -> Lists:
         """Tile lists without a host synchronisation in steady state.  The
-        capacity follows the pair counts seen so far (read back
-        asynchronously); the first call, with no history, reads the count
-        once.  If a call overflows, the raster kernels walk every primitive for
-        that call (exact, slower) and the next call uses a larger buffer."""
+        capacity follows the pair counts seen so far, read back asynchronously
+        every ``READBACK_EVERY`` calls (each read-back is a device-to-host copy
+        on the stream); the first call, with no history, reads the count once.
+        If a call overflows, the raster kernels walk every primitive for that
+        call (exact, slower) until a read-back grows the buffer."""
         dev = ext.device
         N = prims.num_primitives
+        self._lazy_calls += 1
         if self._pending is not None and self._pending[0].query():
             seen = int(self._pending[1][0])
             if seen > 0.9 * self.pair_capacity:
@@ -401,7 +406,7 @@ class Handle:
         cnt = torch.empty(1, dtype=torch.int64, device=dev)
         rg = self._ranges(prims, dev, factor)
         self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
-        if self._pending is None:
+        if self._pending is None and self._lazy_calls % self.READBACK_EVERY == 0:
             host = torch.empty(1, dtype=torch.int64, pin_memory=True)
             host.copy_(cnt, non_blocking=True)
             ev = torch.cuda.Event()

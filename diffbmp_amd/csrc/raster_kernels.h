@@ -628,8 +628,12 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
 }
 
 // Backward with the background composite + MSE as its per-pixel prologue.
+// Without colour blending it is held to 80 VGPRs, 6 waves per SIMD instead of
+// 5; the few values that then spill are read once per staged batch, not per
+// list entry, and the extra wave is worth more (measured).  The blending
+// variants keep their natural allocation (they would spill in the loop).
 template <int DT, bool BLEND>
-static __global__ __launch_bounds__(kThreads) void k_raster_bwd_mse(RasterArgs A, LossArgs L,
+static __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(BLEND ? 1 : 6))) void k_raster_bwd_mse(RasterArgs A, LossArgs L,
                                                                     const float* __restrict__ Cfin,
                                                                     const float* __restrict__ Tfin,
                                                                     const int* __restrict__ lastin,

@@ -116,6 +116,35 @@ def test_overflow_fallback_is_exact(dev):
         assert np.linalg.norm(t.cpu().numpy() - ref) <= 1e-5 * max(np.linalg.norm(ref), 1e-30), k
 
 
+def test_lazy_binning_overflow_then_grows(dev):
+    """Lazy binning with too small a pair buffer: every call still gives the
+    exact image (the raster kernels walk all primitives), and the capacity
+    grows once an asynchronous read-back (one per READBACK_EVERY calls) has
+    seen the count."""
+    from diffbmp_amd import _native
+    sc = S.make_scene(0, with_target=False)
+    run = GpuRun(sc, dev, eps_T=1e-6)
+    col, alp = run.forward()
+    h, shape = run.h, tuple(run.d["templates"].shape)
+    P = h.bin(run.prims, run.ext, shape, h.bin_factor)[1].numel()   # pairs of the lists bin_lazy builds
+    H, W = sc.H, sc.W
+    h.pair_capacity = P // 4
+    grown_at = None
+    for call in range(1, 3 * _native.Handle.READBACK_EVERY + 1):
+        lists = h.bin_lazy(run.prims, run.ext, shape, h.bin_factor)
+        c2 = torch.empty((H, W, 3), device=dev); a2 = torch.empty((H, W), device=dev)
+        T2 = torch.empty((H, W), device=dev); l2 = torch.empty((H, W), dtype=torch.int32, device=dev)
+        h.forward(run.prims, run.bm, lists, 1e-6, c2, a2, T2, l2)
+        torch.cuda.synchronize()
+        assert np.abs(c2.cpu().numpy() - col).max() <= 1e-6, call
+        assert np.abs(a2.cpu().numpy() - alp).max() <= 1e-6, call
+        if grown_at is None and h.pair_capacity >= P:
+            grown_at = call
+    print("capacity grown at call", grown_at, "of", 3 * _native.Handle.READBACK_EVERY)
+    assert grown_at is not None and grown_at <= 2 * _native.Handle.READBACK_EVERY + 1
+    assert int(lists.count.item()) <= lists.capacity
+
+
 @pytest.mark.parametrize("H,W,ts", [(100, 70, 16), (96, 160, 32), (17, 33, 16)])
 def test_ragged_canvas_and_tile_sizes(dev, H, W, ts):
     sc = S.make_scene(0, H=H, W=W, N=40, tsz=32, with_target=False)
