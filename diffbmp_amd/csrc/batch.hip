@@ -369,6 +369,8 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.ranges = nullptr;
     A.order = nullptr;
     A.cost = nullptr;
+    A.cull = nullptr;
+    A.ntmpl = 0;
     A.bin_shift = 0;
     d.alpha_ub = s.alpha_upper_bound;
     d.L.target = s.target;

@@ -29,7 +29,11 @@ constexpr int kRadixTile = kRadixThreads * kRadixItems;  // keys per block
 
 // Template extents (vector_renderer.py:151-192): min/max column/row of texels
 // > 1e-6 mapped through u = (col/(W-1))*2 - 1 in fp32; empty => (-1,1,-1,1).
-template <typename TT>
+// With BOX the output is instead each template's content box for the raster
+// kernels' culling, float4 (u centre, v centre, u half-width, v half-height),
+// widened by one texel on every side: a bilinear sample outside it reads four
+// texels <= 1e-6, so its mask is <= 1e-6 and the sample is never pushed.
+template <typename TT, bool BOX = false>
 static __global__ __launch_bounds__(256) void k_template_extents(const TT* __restrict__ tmpl, int Ht,
                                                           int Wt, float* __restrict__ ext) {
   const int t = blockIdx.x;
@@ -48,7 +52,18 @@ static __global__ __launch_bounds__(256) void k_template_extents(const TT* __res
   atomicMin(&red[0], cmin); atomicMax(&red[1], cmax);
   atomicMin(&red[2], rmin); atomicMax(&red[3], rmax);
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (BOX && threadIdx.x == 0) {
+    const float du = 2.f / (float)(Wt - 1), dv = 2.f / (float)(Ht - 1);
+    float u0 = -1.f - du, u1 = 1.f + du, v0 = -1.f - dv, v1 = 1.f + dv;
+    if (red[1] >= 0) {
+      u0 = (float)(red[0] - 1) * du - 1.f;
+      u1 = (float)(red[1] + 1) * du - 1.f;
+      v0 = (float)(red[2] - 1) * dv - 1.f;
+      v1 = (float)(red[3] + 1) * dv - 1.f;
+    }
+    reinterpret_cast<float4*>(ext)[t] =
+        make_float4(0.5f * (u0 + u1), 0.5f * (v0 + v1), 0.5f * (u1 - u0), 0.5f * (v1 - v0));
+  } else if (threadIdx.x == 0) {
     float* e = ext + 4 * t;
     if (red[1] < 0) {
       e[0] = -1.f; e[1] = 1.f; e[2] = -1.f; e[3] = 1.f;

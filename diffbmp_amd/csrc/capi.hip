@@ -84,6 +84,9 @@ struct dbmp_rasterizer {
   bool q_rgba = false;
   const void* q_csrc = nullptr;
   long long q_cversion = -2;
+  // per-template content boxes of the templates the quads hold (culling)
+  int cull_cap = 0;
+  float4* cull = nullptr;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_fwd, ev_bwd;
@@ -218,6 +221,22 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   }
   int rc = rgba ? launch_build_rgba(b, h->quads, st) : launch_build_quads(b, h->quads, st);
   if (rc) return rc;
+  // content boxes from the same texels the quads hold
+  if (b->num_templates > h->cull_cap) {
+    if ((rc = grow(&h->cull, b->num_templates))) return rc;
+    h->cull_cap = b->num_templates;
+  }
+  const dim3 cgrid(b->num_templates), cblock(256);
+  if (b->dtype == DBMP_F32)
+    hipLaunchKernelGGL((k_template_extents<float, true>), cgrid, cblock, 0, st,
+                       (const float*)b->templates, b->height, b->width, (float*)h->cull);
+  else if (b->dtype == DBMP_F16)
+    hipLaunchKernelGGL((k_template_extents<__half, true>), cgrid, cblock, 0, st,
+                       (const __half*)b->templates, b->height, b->width, (float*)h->cull);
+  else
+    hipLaunchKernelGGL((k_template_extents<__hip_bfloat16, true>), cgrid, cblock, 0, st,
+                       (const __hip_bfloat16*)b->templates, b->height, b->width, (float*)h->cull);
+  LAUNCH_CHECK("k_template_extents");
   h->q_src = b->templates;
   h->q_version = b->version;
   h->q_T = b->num_templates;
@@ -249,6 +268,8 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.bin_shift = 0;
   A.order = nullptr;
   A.cost = nullptr;
+  A.cull = h->cull;   // built with the quads (ensure_quads runs first)
+  A.ntmpl = b->num_templates;
   A.c_blend = b->c_blend;
   A.exact_blend = b->blend_grad_exact != 0;
   A.N = p->num_primitives;
@@ -370,7 +391,7 @@ int dbmp_destroy(dbmp_rasterizer* h) {
   timing_drain(h->ev_bwd, ms, n);
   void* ptrs[] = {h->recs, h->counts, h->offs, h->ranges, h->grad8, h->keysA,
                   h->keysB, h->valsB, h->hist, h->totals, h->quads, h->blk_cost, h->blk_order,
-                  h->tcount4};
+                  h->tcount4, h->cull};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;

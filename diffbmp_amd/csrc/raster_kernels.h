@@ -55,6 +55,8 @@ struct RasterArgs {
   int bin_shift;         // log2(list tile / 16) when ranges is set
   const int* order;      // workgroup -> block, heaviest first (k_block_order), or null
   int* cost;             // forward: each block's walk cost, or null
+  const float4* cull;    // per template: content box (k_template_extents<_, true>), or null
+  int ntmpl;             // templates in `cull`
 };
 
 // Block a workgroup runs: the heaviest-first order of the last forward's
@@ -189,20 +191,59 @@ __device__ __forceinline__ float4 fetch_rec_part(const RasterArgs& A, bool all, 
   return part == 0 ? make_float4(0.f, 0.f, -1.f, -1.f) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// A wave-uniform float kept in a scalar register (block constants that the
+// compiler hoists out of the batch loop would otherwise hold vector registers).
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // Bit w set when the push box (quarter 0 of the record: mx, my, hw, hh) can
 // touch wave w's pixels [bx, bx+15] x [by+4w, by+4w+3]; 1 px slack keeps the
-// test conservative w.r.t. the per-pixel fp32 test.
+// test conservative w.r.t. the per-pixel fp32 test.  The block bounds are
+// recomputed at each call (the empty asm hides that they are loop invariant)
+// and kept in scalar registers: hoisted out of the batch loop they would hold
+// vector registers across it and cost the raster loops occupancy.
 __device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
+  __asm__ volatile("" : "+s"(bx), "+s"(by));
   const float mx = q.x, my = q.y, hw = q.z, hh = q.w;
   if (!(hw >= 0.f && hh >= 0.f)) return 0;
-  if (mx - hw > (float)(bx + 16) || mx + hw < (float)(bx - 1)) return 0;
+  if (mx - hw > uniform_f((float)(bx + 16)) || mx + hw < uniform_f((float)(bx - 1))) return 0;
   int m = 0;
 #pragma unroll
   for (int w = 0; w < 4; ++w) {
-    const float y0 = (float)(by + 4 * w);
-    if (!(my - hh > y0 + 4.f || my + hh < y0 - 1.f)) m |= 1 << w;
+    const int y0 = by + 4 * w;
+    if (!(my - hh > uniform_f((float)(y0 + 4)) || my + hh < uniform_f((float)(y0 - 1)))) m |= 1 << w;
   }
   return m;
+}
+
+// Whether the content of a staged entry's template can reach the wave's
+// 16x4 pixels centred on (cx, cy): separating-axis test, on the axes of the
+// primitive's rotated content box (its template's texels above the mask
+// threshold, one texel wider: outside it no sample is pushed), of that box
+// against the wave's pixel rectangle, with one pixel of slack.  The push box
+// (wave_mask) covers the canvas axes.  An entry it rejects has no pushed
+// sample in the wave, so skipping it leaves every result unchanged.
+__device__ __forceinline__ bool content_hits(const RasterArgs& A, const PrimRec& p, float cx,
+                                             float cy) {
+  const int t = p.tmpl;
+  const float4 b = A.cull[min(max(t, 0), A.ntmpl - 1)];
+  const float ac = fabsf(p.c), as = fabsf(p.s);
+  const float dx = cx - p.mx, dy = cy - p.my;
+  const float pu = fmaf(p.c, dx, p.s * dy) - p.r * b.x;
+  const float pv = fmaf(p.c, dy, -p.s * dx) - p.r * b.y;
+  const float eu = fmaf(p.r, b.z, fmaf(ac, 7.5f, fmaf(as, 1.5f, 1.f)));
+  const float ev = fmaf(p.r, b.w, fmaf(as, 7.5f, fmaf(ac, 1.5f, 1.f)));
+  return ((unsigned)t < (unsigned)A.ntmpl) & (fabsf(pu) <= eu) & (fabsf(pv) <= ev);
+}
+
+// Entries of staged batch `buf` that can touch this wave's pixels.
+__device__ __forceinline__ bool entry_hits_wave(const RasterArgs& A, const PrimRec* srec,
+                                                const int* smask, int lane, int wbit, float cx,
+                                                float cy) {
+  bool in = (smask[lane] & wbit) != 0;
+  if (A.cull != nullptr) in = in & content_hits(A, srec[lane], cx, cy);
+  return in;
 }
 
 // Lane-mask vote on a bool: the ballot stays a scalar mask (no per-lane 0/1
@@ -324,17 +365,23 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   const bool all = list_range(A, G.tile, start, end);
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
   const int wbit = 1 << G.w;
+  const float ccx = uniform_f((float)G.bx + 7.5f);   // centre of the wave's pixels
+  const float ccy = uniform_f((float)(G.by + 4 * G.w) + 1.5f);
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
   __syncthreads();
   int buf = 0;
   for (int base = start; base < end; base += kBatch, buf ^= 1) {
     const bool have_next = base + kBatch < end;
-    Stage nxt;
     ++batches;
+    // entries of this batch that can touch this wave's pixels (before the
+    // next batch's loads are in flight: fewer live registers)
+    const bool any_alive = wave_any(alive);
+    unsigned long long wm = 0ull;
+    if (any_alive)
+      wm = wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wbit, ccx, ccy));
+    Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
-    if (wave_any(alive)) {
-      // entries of this batch whose push box can touch this wave's rows
-      unsigned long long wm = wave_ballot((smask[buf][G.lane] & wbit) != 0);
+    if (any_alive) {
       while (wm) {
         const int j = __builtin_ctzll(wm);
         wm &= wm - 1ull;
@@ -405,8 +452,10 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   }
 }
 
+// Without colour blending the forward fits 64 VGPRs: 8 waves per SIMD, the
+// most a SIMD holds (the attribute keeps it there).
 template <int DT, bool BLEND>
-__global__ __launch_bounds__(kThreads) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(BLEND ? 1 : 8))) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
                                                          float* __restrict__ out_alpha,
                                                          float* __restrict__ out_T,
                                                          int* __restrict__ out_last) {
@@ -491,20 +540,24 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   if (G.lane == 0) atomicMax(&skmax, wkmax);
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
   const int wbit = 1 << G.w;
+  const float ccx = uniform_f((float)G.bx + 7.5f);   // centre of the wave's pixels
+  const float ccy = uniform_f((float)(G.by + 4 * G.w) + 1.5f);
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
   __syncthreads();
   const int stop = start + min(skmax, end - start);
   int buf = 0;
   for (int base = start; base < stop; base += kBatch, buf ^= 1) {
     const bool have_next = base + kBatch < stop;
-    Stage nxt;
-    if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
     const int cnt = __builtin_amdgcn_readfirstlane(min(kBatch, stop - base));
-    // entries of this batch that touch this wave's rows, up to the wave's
-    // furthest forward position
-    unsigned long long wm = wave_ballot((smask[buf][G.lane] & wbit) != 0);
+    // entries of this batch that can touch this wave's pixels, up to the
+    // wave's furthest forward position (before the next batch's loads are
+    // in flight: fewer live registers)
+    unsigned long long wm =
+        wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wbit, ccx, ccy));
     const int lim = wkmax - (base - start);
     wm = lim >= 64 ? wm : (lim <= 0 ? 0ull : wm & ((1ull << lim) - 1ull));
+    Stage nxt;
+    if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
     while (wm) {
       const int j = __builtin_ctzll(wm);
       wm &= wm - 1ull;
