@@ -73,6 +73,7 @@ struct dbmp_rasterizer {
   int* blk_cost = nullptr;
   int* blk_order = nullptr;
   int order_n = 0;
+  int order_age = 0;          // forwards since the order was last rebuilt
   // texel quads cache
   void* quads = nullptr;
   size_t quads_bytes = 0;     // allocated
@@ -139,6 +140,7 @@ int ensure_blocks(dbmp_rasterizer* h, int nb) {
 // L2.  Larger bitmaps are L2-bound: there the natural order, which runs
 // neighbouring tiles (sharing texels) together, is faster.
 constexpr size_t kOrderMaxQuadBytes = 4u << 20;
+constexpr int kOrderEvery = 4;
 bool uses_block_order(const dbmp_rasterizer* h) { return h->quads_used <= kOrderMaxQuadBytes; }
 const int* block_order(const dbmp_rasterizer* h, int nb) {
   return (h->order_n == nb && uses_block_order(h)) ? h->blk_order : nullptr;
@@ -542,19 +544,25 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
   RasterArgs A = make_args(h, prims, bm, lists, eps_T);
   const int nb = A.tiles * A.subs;
   if ((rc = ensure_blocks(h, nb))) return rc;
-  const bool ordered = uses_block_order(h);
+  // the order is rebuilt from this forward's costs every kOrderEvery forwards
+  // (costs change slowly from one call to the next; the order only balances
+  // the launch, every order gives the same results)
+  const bool refresh = uses_block_order(h) && (h->order_n != nb || h->order_age + 1 >= kOrderEvery);
   A.order = block_order(h, nb);
-  A.cost = ordered ? h->blk_cost : nullptr;
+  A.cost = refresh ? h->blk_cost : nullptr;
   const bool blend = A.cmap != nullptr;
   DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(nb), dim3(kThreads), 0, st, A,
                     out_color, out_alpha, out_T_final, out_last);
   LAUNCH_CHECK("k_raster_fwd");
-  if (ordered) {
-    // this forward's costs order the backward and the next forward
+  if (refresh) {
+    // this forward's costs order the backward and the next forwards
     hipLaunchKernelGGL(k_block_order, dim3(1), dim3(kOrderThreads), 0, st, h->blk_cost, nb,
                        h->blk_order);
     LAUNCH_CHECK("k_block_order");
     h->order_n = nb;
+    h->order_age = 0;
+  } else {
+    ++h->order_age;
   }
   timing_end(h, h->ev_fwd, st);
   return 0;
