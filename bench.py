@@ -38,7 +38,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--precision", default=None, choices=["fp32", "fp16", "bf16"],
+                    help="parameter / bitmap storage (arithmetic is fp32); default fp32, fp16 for "
+                         "--workload stress")
     ap.add_argument("--eps-t", type=float, default=None)
     ap.add_argument("--mode", default="fused", choices=["fused", "autograd"],
                     help="fused: the device-resident FitJob step (binning, forward, loss, backward, "
@@ -63,7 +65,10 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "reference kernel for scaling HBM byte counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "fp16" if args.workload == "stress" else "fp32"
+    return args
 
 
 def algorithmic_bytes(H, W, N, T, Ht, Wt, P, tiles, b_p, b_t):
