@@ -78,6 +78,8 @@ void drop_graph(dbmp_batch* b) {
 int check_job(const dbmp_job& j, int i) {
   if (j.height <= 0 || j.width <= 0) return fail(DBMP_E_ARG, "job %d: canvas must be positive", i);
   if (j.num_primitives < 0) return fail(DBMP_E_ARG, "job %d: num_primitives < 0", i);
+  if (!(j.alpha_upper_bound >= 0.f))
+    return fail(DBMP_E_ARG, "job %d: alpha_upper_bound must be >= 0", i);
   if (j.num_templates <= 0 || j.tmpl_height < 2 || j.tmpl_width < 2)
     return fail(DBMP_E_ARG, "job %d: bad template shape", i);
   if ((long long)j.num_templates * j.tmpl_height * j.tmpl_width >= (1LL << 28))

@@ -359,6 +359,8 @@ int dbmp_init(dbmp_rasterizer** out, int image_height, int image_width, int tile
   *out = nullptr;
   if (image_height <= 0 || image_width <= 0)
     return fail(DBMP_E_ARG, "image size must be positive, got %dx%d", image_height, image_width);
+  if (!(alpha_upper_bound >= 0.f))
+    return fail(DBMP_E_ARG, "alpha_upper_bound must be >= 0, got %g", (double)alpha_upper_bound);
   if (tile_size <= 0 || tile_size % kSub != 0)
     return fail(DBMP_E_ARG, "tile_size must be a positive multiple of %d, got %d", kSub, tile_size);
   auto* h = new dbmp_rasterizer();

@@ -353,9 +353,12 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   }
   const float fpx = (float)G.px, fpy = (float)G.py;
   bool alive = (G.px < A.W) && (G.py < A.H);
-  // T: transmittance; Tnz: product of the nonzero (1 - a) factors; nz: count
-  // of zero factors (exactly opaque layers), capped at 2
-  float T = 1.f, Tnz = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+  // Tnz: product of the nonzero (1 - a) factors; nz: count of zero factors
+  // (exactly opaque layers), capped at 2.  Until a lane meets an exactly
+  // opaque layer Tnz is its transmittance T; from then on T is 0, the lane's
+  // colour is the snapshot Cf taken at that layer, and the C0..C2 it keeps
+  // accumulating are dropped.  One running product instead of two.
+  float Tnz = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, Cf0 = 0.f, Cf1 = 0.f, Cf2 = 0.f;
   int nz = 0;
   float thr = alive ? A.eps : -1.f;   // stop test threshold (below)
   int last = -1;
@@ -401,23 +404,27 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
           col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
         }
         const float a = S.pushed ? p.abase * S.m : 0.f;   // a = 0: compositing is a no-op
-        const float wgt = T * a;
+        const float wgt = Tnz * a;
         C0 = fmaf(wgt, col[0], C0);
         C1 = fmaf(wgt, col[1], C1);
         C2 = fmaf(wgt, col[2], C2);
-        const float f = fmaxf(1.f - a, 0.f);
-        T = T * f;
+        // max(1 - a, 0) as one clamped subtraction (a >= 0: alpha_ub >= 0)
+        const float f = fminf(fmaxf(1.f - a, 0.f), 1.f);
         // a pixel stops once later layers can change neither its colour nor
         // its alpha (or alpha's derivative w.r.t. an opaque layer) by eps_T
         // (Tnz < thr: thr is eps_T on a live lane, +inf once it has seen two
         // opaque layers, -1 on a stopped lane; a live lane that is not pushed
         // keeps Tnz >= eps_T, since eps_T < 1, so the test needs no `pushed`)
-        // Tnz keeps the nonzero factors; nz counts exactly opaque layers
-        // (capped at 2), rare enough for a wave-uniform branch
+        // nz counts exactly opaque layers (capped at 2), rare enough for a
+        // wave-uniform branch
         const bool opq = f == 0.f;
         Tnz = opq ? Tnz : Tnz * f;
         if (wave_any(opq)) {
           __asm__ volatile("");   // keeps this rare block a branch, not selects
+          const bool first = opq & (nz == 0);
+          Cf0 = first ? C0 : Cf0;
+          Cf1 = first ? C1 : Cf1;
+          Cf2 = first ? C2 : Cf2;
           nz = opq ? min(nz + 1, 2) : nz;
           thr = (alive & (nz >= 2)) ? __builtin_inff() : thr;
         }
@@ -435,10 +442,10 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   }
   if (G.px < A.W && G.py < A.H) {
     const long p = (long)G.py * A.W + G.px;
-    out_color[3 * p + 0] = C0;
-    out_color[3 * p + 1] = C1;
-    out_color[3 * p + 2] = C2;
-    out_alpha[p] = 1.f - T;
+    out_color[3 * p + 0] = nz ? Cf0 : C0;
+    out_color[3 * p + 1] = nz ? Cf1 : C1;
+    out_color[3 * p + 2] = nz ? Cf2 : C2;
+    out_alpha[p] = 1.f - (nz ? 0.f : Tnz);
     out_T[p] = Tnz;
     out_last[p] = (last < 0 ? end - start : last) | (nz << 30);
   }
@@ -584,10 +591,10 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       ca2 = fmaf(wgt, col[2], ca2);
       const float gS = gcf - fmaf(gc2, ca2, fmaf(gc1, ca1, gc0 * ca0));
       const float gcol = fmaf(gc2, col[2], fmaf(gc1, col[1], gc0 * col[0]));
-      const float om = 1.f - a;
-      // max(om, 1e-6) as one med3 (om <= 1 - 0 for a >= 0; FLT_MAX bounds it otherwise)
-      const float inv1m = __builtin_amdgcn_rcpf(__builtin_amdgcn_fmed3f(om, 1e-6f, 3.402823466e38f));
-      const float fk = fmaxf(om, 0.f);
+      // fk = max(1 - a, 0) as one clamped subtraction (a >= 0: alpha_ub >= 0);
+      // 1 / max(1 - a, 1e-6) = 1 / max(fk, 1e-6)
+      const float fk = fminf(fmaxf(1.f - a, 0.f), 1.f);
+      const float inv1m = __builtin_amdgcn_rcpf(fmaxf(fk, 1e-6f));
       const float Tn = Tk * fk;
       // T_k B_k, with the suffix transmittance B_k = prod_{m>k} (1 - a_m) =
       // T_final / T_{k+1}: T_k B_k = T_final / (1 - a_k) while 1 - a_k >= 1e-6
@@ -596,7 +603,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       // nonzero-factor product after it, tnzf / T_k, if it is the only opaque
       // layer, else 0
       float TkB;
-      if (wave_any(om < 1e-6f)) {
+      if (wave_any(fk < 1e-6f)) {
         const bool tpos = Tn > 0.f;
         const float rB = fminf((tpos ? tf : tnzf) * __builtin_amdgcn_rcpf(tpos ? Tn : Tk), 1.f);
         TkB = Tk * (tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f));

@@ -99,7 +99,9 @@ const char* dbmp_last_error(void);
  * tile_size must be a positive multiple of 16.  sigma and
  * max_prims_per_pixel are accepted for signature parity and not used: the
  * reference never reads sigma in its kernels, and this implementation keeps
- * no per-pixel layer cache, so it has no per-pixel capacity limit.  A
+ * no per-pixel layer cache, so it has no per-pixel capacity limit.
+ * alpha_upper_bound must be >= 0 (an opacity bound; the kernels rely on
+ * alpha >= 0, and a negative bound is rejected).  A
  * handle's scratch (primitive records, sort buffers, texel records, block
  * costs and launch order) is shared by its calls: use a handle from one
  * stream at a time. */
@@ -279,7 +281,7 @@ typedef struct {
   float* loss_history;          /* (loss_capacity) MSE of step t at [t], zeroed by the caller */
   int32_t loss_capacity;
   float lr[5];                  /* means2D, radii, rotations, opacities, colors */
-  float alpha_upper_bound;
+  float alpha_upper_bound;      /* >= 0 */
   float eps_T;
   int32_t clamp;                /* x in [0,W-1], y in [0,H-1], r in [2, min(H,W)/4] */
   int32_t steps_done;           /* Adam steps already taken (the next step is steps_done+1) */

@@ -43,6 +43,9 @@ def test_version_and_argument_errors_without_device():
     assert b"multiple of 16" in L.dbmp_last_error()
     assert L.dbmp_init(ctypes.byref(h), 0, 64, 16, 0.0, 1.0, 151, 0) == 1
     assert L.dbmp_init(None, 64, 64, 16, 0.0, 1.0, 151, 0) == 1
+    assert L.dbmp_init(ctypes.byref(h), 64, 64, 16, 0.0, -0.5, 151, 0) == 1
+    assert b"alpha_upper_bound" in L.dbmp_last_error()
+    assert L.dbmp_init(ctypes.byref(h), 64, 64, 16, 0.0, float("nan"), 151, 0) == 1
     assert L.dbmp_forward(None, None, None, None, 0.0, None, None, None, None, None) == 1
     assert L.dbmp_destroy(None) == 0
     b = ctypes.c_void_p()
