@@ -2,11 +2,13 @@
 //
 // Work decomposition: one 256-thread workgroup (4 wave64s) owns a 16x16 pixel
 // block ("sub-tile") and walks the tile list of the binning tile containing
-// it; wave w owns pixel rows 4w..4w+3, one pixel per lane.  The list is
+// it; wave w owns the 8x8 quadrant (w & 1, w >> 1), one pixel per lane (a
+// square footprint meets fewer primitive edges than a 16x4 strip: fewer
+// entries per wave pass the cull below).  The list is
 // consumed in batches of 64 entries whose 64-byte primitive records are
 // staged in LDS, double buffered: while the waves composite batch b, the
 // records of batch b+1 are already in flight from L2 into registers.  Each
-// wave culls a list entry against its own 16x4 pixel rectangle and stops as
+// wave culls a list entry against its own 8x8 pixel square and stops as
 // soon as every lane's transmittance is below eps_T; the workgroup stops when
 // all four waves have.
 //
@@ -162,8 +164,8 @@ __device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A, int b) {
   // cull tests and loop counters in scalar registers
   g.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   g.lane = threadIdx.x & 63;
-  g.px = g.bx + (g.lane & 15);
-  g.py = g.by + 4 * g.w + (g.lane >> 4);
+  g.px = g.bx + 8 * (g.w & 1) + (g.lane & 7);
+  g.py = g.by + 8 * (g.w >> 1) + (g.lane >> 3);
   return g;
 }
 
@@ -198,8 +200,9 @@ __device__ __forceinline__ float uniform_f(float v) {
 }
 
 // Bit w set when the push box (quarter 0 of the record: mx, my, hw, hh) can
-// touch wave w's pixels [bx, bx+15] x [by+4w, by+4w+3]; 1 px slack keeps the
-// test conservative w.r.t. the per-pixel fp32 test.  The block bounds are
+// touch wave w's pixels [x0, x0+7] x [y0, y0+7], x0 = bx + 8 (w & 1),
+// y0 = by + 8 (w >> 1); 1 px slack keeps the test conservative w.r.t. the
+// per-pixel fp32 test.  The block bounds are
 // recomputed at each call (the empty asm hides that they are loop invariant)
 // and kept in scalar registers: hoisted out of the batch loop they would hold
 // vector registers across it and cost the raster loops occupancy.
@@ -208,17 +211,15 @@ __device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
   const float mx = q.x, my = q.y, hw = q.z, hh = q.w;
   if (!(hw >= 0.f && hh >= 0.f)) return 0;
   if (mx - hw > uniform_f((float)(bx + 16)) || mx + hw < uniform_f((float)(bx - 1))) return 0;
-  int m = 0;
-#pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    const int y0 = by + 4 * w;
-    if (!(my - hh > uniform_f((float)(y0 + 4)) || my + hh < uniform_f((float)(y0 - 1)))) m |= 1 << w;
-  }
-  return m;
+  const bool xl = !(mx - hw > uniform_f((float)(bx + 8)) || mx + hw < uniform_f((float)(bx - 1)));
+  const bool xr = !(mx - hw > uniform_f((float)(bx + 16)) || mx + hw < uniform_f((float)(bx + 7)));
+  const bool yt = !(my - hh > uniform_f((float)(by + 8)) || my + hh < uniform_f((float)(by - 1)));
+  const bool yb = !(my - hh > uniform_f((float)(by + 16)) || my + hh < uniform_f((float)(by + 7)));
+  return (int)(xl & yt) | ((int)(xr & yt) << 1) | ((int)(xl & yb) << 2) | ((int)(xr & yb) << 3);
 }
 
 // Whether the content of a staged entry's template can reach the wave's
-// 16x4 pixels centred on (cx, cy): separating-axis test, on the axes of the
+// 8x8 pixels centred on (cx, cy): separating-axis test, on the axes of the
 // primitive's rotated content box (its template's texels above the mask
 // threshold, one texel wider: outside it no sample is pushed), of that box
 // against the wave's pixel rectangle, with one pixel of slack.  The push box
@@ -232,8 +233,9 @@ __device__ __forceinline__ bool content_hits(const RasterArgs& A, const PrimRec&
   const float dx = cx - p.mx, dy = cy - p.my;
   const float pu = fmaf(p.c, dx, p.s * dy) - p.r * b.x;
   const float pv = fmaf(p.c, dy, -p.s * dx) - p.r * b.y;
-  const float eu = fmaf(p.r, b.z, fmaf(ac, 7.5f, fmaf(as, 1.5f, 1.f)));
-  const float ev = fmaf(p.r, b.w, fmaf(as, 7.5f, fmaf(ac, 1.5f, 1.f)));
+  const float k = fmaf(ac + as, 3.5f, 1.f);   // the square's projection (half side 3.5) + slack
+  const float eu = fmaf(p.r, b.z, k);
+  const float ev = fmaf(p.r, b.w, k);
   return ((unsigned)t < (unsigned)A.ntmpl) & (fabsf(pu) <= eu) & (fabsf(pv) <= ev);
 }
 
@@ -368,8 +370,8 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   const bool all = list_range(A, G.tile, start, end);
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
   const int wbit = 1 << G.w;
-  const float ccx = uniform_f((float)G.bx + 7.5f);   // centre of the wave's pixels
-  const float ccy = uniform_f((float)(G.by + 4 * G.w) + 1.5f);
+  const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
+  const float ccy = uniform_f((float)(G.by + 8 * (G.w >> 1)) + 3.5f);
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
   __syncthreads();
   int buf = 0;
@@ -547,8 +549,8 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   if (G.lane == 0) atomicMax(&skmax, wkmax);
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
   const int wbit = 1 << G.w;
-  const float ccx = uniform_f((float)G.bx + 7.5f);   // centre of the wave's pixels
-  const float ccy = uniform_f((float)(G.by + 4 * G.w) + 1.5f);
+  const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
+  const float ccy = uniform_f((float)(G.by + 8 * (G.w >> 1)) + 3.5f);
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
   __syncthreads();
   const int stop = start + min(skmax, end - start);
