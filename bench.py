@@ -31,6 +31,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK = 8.0e12
+# wave64 VALU instructions per second: 256 CUs x 2.4 GHz x one instruction per
+# CU per cycle (4 SIMDs of 16 lanes, 4 cycles per wave64 instruction, no
+# packed fp32)
+VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
 def parse():
@@ -500,6 +504,20 @@ def main():
                      "kernel": bwd_kernel, "algorithmic_bytes": int(b_bwd),
                      "step_algorithmic_bytes": int(b_bin + b_fwd + b_bwd)},
     }
+    # the bound the raster kernels actually sit at: VALU instruction issue
+    # (instruction count per launch from the committed SQ_INSTS_VALU pass of
+    # this workload, divided by the live hipEvent duration)
+    vprof = os.path.join(HERE, "profiles", "pmc_valu_scale_fp32.json")
+    if args.precision == "fp32" and os.path.exists(vprof):
+        with open(vprof) as f:
+            vk = json.load(f)["kernels"].get(bwd_kernel)
+        if vk:
+            ach = vk["SQ_INSTS_VALU"] / res["bwd_s"]
+            out["issue_roofline"] = {"bound": "valu_issue", "achieved": round(ach / 1e9, 2),
+                                     "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave64 VALU instr/s",
+                                     "frac": round(ach / VALU_ISSUE_PEAK, 4), "kernel": bwd_kernel,
+                                     "valu_instructions": vk["SQ_INSTS_VALU"],
+                                     "source": "profiles/pmc_valu_scale_fp32.json"}
     if args.psnr_steps:
         out["psnr"] = psnr_compare(args, sc, dev, sdt)
     if not args.no_cpu_baseline:

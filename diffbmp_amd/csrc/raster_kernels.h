@@ -193,28 +193,27 @@ __device__ __forceinline__ float4 fetch_rec_part(const RasterArgs& A, bool all, 
   return part == 0 ? make_float4(0.f, 0.f, -1.f, -1.f) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// A wave-uniform float kept in a scalar register (block constants that the
-// compiler hoists out of the batch loop would otherwise hold vector registers).
+// A wave-uniform float kept in a scalar register.
 __device__ __forceinline__ float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
 // Bit w set when the push box (quarter 0 of the record: mx, my, hw, hh) can
 // touch wave w's pixels [x0, x0+7] x [y0, y0+7], x0 = bx + 8 (w & 1),
-// y0 = by + 8 (w >> 1); 1 px slack keeps the test conservative w.r.t. the
-// per-pixel fp32 test.  The block bounds are
-// recomputed at each call (the empty asm hides that they are loop invariant)
-// and kept in scalar registers: hoisted out of the batch loop they would hold
-// vector registers across it and cost the raster loops occupancy.
+// y0 = by + 8 (w >> 1).  Conservative w.r.t. the per-pixel fp32 test: the
+// box is widened to whole pixels (floor of its low edges, floor + 1 of its
+// high edges) and compared in integers against the block's scalar bounds
+// (float-to-int conversion saturates; no +1 on a converted value, so no
+// overflow).
 __device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
-  __asm__ volatile("" : "+s"(bx), "+s"(by));
   const float mx = q.x, my = q.y, hw = q.z, hh = q.w;
   if (!(hw >= 0.f && hh >= 0.f)) return 0;
-  if (mx - hw > uniform_f((float)(bx + 16)) || mx + hw < uniform_f((float)(bx - 1))) return 0;
-  const bool xl = !(mx - hw > uniform_f((float)(bx + 8)) || mx + hw < uniform_f((float)(bx - 1)));
-  const bool xr = !(mx - hw > uniform_f((float)(bx + 16)) || mx + hw < uniform_f((float)(bx + 7)));
-  const bool yt = !(my - hh > uniform_f((float)(by + 8)) || my + hh < uniform_f((float)(by - 1)));
-  const bool yb = !(my - hh > uniform_f((float)(by + 16)) || my + hh < uniform_f((float)(by + 7)));
+  const int x0 = (int)floorf(mx - hw), x1 = (int)floorf(mx + hw);
+  const int y0 = (int)floorf(my - hh), y1 = (int)floorf(my + hh);
+  const bool xl = (x0 <= bx + 7) & (x1 >= bx - 1);
+  const bool xr = (x0 <= bx + 15) & (x1 >= bx + 7);
+  const bool yt = (y0 <= by + 7) & (y1 >= by - 1);
+  const bool yb = (y0 <= by + 15) & (y1 >= by + 7);
   return (int)(xl & yt) | ((int)(xr & yt) << 1) | ((int)(xl & yb) << 2) | ((int)(xr & yb) << 3);
 }
 
