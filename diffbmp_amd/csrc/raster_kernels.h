@@ -347,6 +347,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   __shared__ PrimRec srec[2][kBatch];
   __shared__ int sid[2][kBatch];
   __shared__ int smask[2][kBatch];
+  __shared__ __align__(16) int salive[2][4];
   const BlockGeom G = block_geom(A, b);
   if (G.bx >= A.W || G.by >= A.H) {   // 16-px block of a tile past the canvas edge
     if (A.cost != nullptr && threadIdx.x == 0) A.cost[b] = 0;
@@ -439,7 +440,14 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
       }
     }
     if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G.bx, G.by);
-    if (!__syncthreads_or(alive)) break;
+    // one barrier: it publishes the staged batch and each wave's liveness
+    // (one flag per wave and buffer parity, so a flag is rewritten only after
+    // every wave has read it; __syncthreads_or takes three barriers)
+    const int wave_alive = wave_any(alive) ? 1 : 0;
+    if (G.lane == 0) salive[buf][G.w] = wave_alive;
+    __syncthreads();
+    const int4 fl = *reinterpret_cast<const int4*>(salive[buf]);
+    if ((fl.x | fl.y | fl.z | fl.w) == 0) break;
   }
   if (G.px < A.W && G.py < A.H) {
     const long p = (long)G.py * A.W + G.px;
