@@ -57,6 +57,8 @@ struct dbmp_batch {
   long long* dtot = nullptr;
   long long* host_total = nullptr;      // mapped pinned mirror of *dtot
   long long* host_total_dev = nullptr;  // its device address
+  long long cull_cap = 0;
+  float4* cull = nullptr;               // per-template content boxes of every job
   // graph of the current set
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -256,7 +258,7 @@ int dbmp_batch_destroy(dbmp_batch* b) {
   if (b->cap_stream) (void)hipStreamSynchronize(b->cap_stream);
   void* ptrs[] = {b->djobs, b->dstate, b->dstarts, b->recs, b->counts, b->offs, b->ranges,
                   b->grad8, b->keysA, b->keysB, b->valsB, b->tidx, b->hist, b->totals, b->toffs,
-                  b->dtot};
+                  b->dtot, b->cull};
   (void)hipDeviceSynchronize();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -307,8 +309,11 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
   hipStream_t st = (hipStream_t)stream;
   const int J = num_jobs;
   std::vector<int> ps(J + 1), ts(J + 1), bs(J + 1);
-  long long Nt = 0, Tt = 0, Bt = 0;
+  std::vector<long long> cs(J + 1);
+  long long Nt = 0, Tt = 0, Bt = 0, Ct = 0;
   for (int i = 0; i < J; ++i) {
+    cs[i] = Ct;
+    Ct += jobs[i].num_templates;
     ps[i] = (int)Nt;
     ts[i] = (int)Tt;
     bs[i] = (int)Bt;
@@ -324,7 +329,7 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     return fail(DBMP_E_ARG, "job set too large (%lld primitives, %lld tiles)", Nt, Tt);
   // every buffer the previous steps may still read stays alive until they end
   const bool grows = Nt > b->n_cap || pair_capacity > b->pair_alloc || Tt + 1 > b->tile_alloc ||
-                     J > b->job_cap;
+                     J > b->job_cap || Ct > b->cull_cap;
   if (grows) HIP_TRY(hipStreamSynchronize(st));
   if ((rc = ensure_prims(b, (int)std::max<long long>(Nt, 1)))) return rc;
   if ((rc = ensure_pairs(b, pair_capacity))) return rc;
@@ -333,6 +338,10 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     b->tile_alloc = (int)(Tt + 1);
   }
   if ((rc = ensure_jobs(b, std::max(J, 1)))) return rc;
+  if (Ct > b->cull_cap) {
+    if ((rc = grow(&b->cull, Ct))) return rc;
+    b->cull_cap = Ct;
+  }
 
   std::vector<BatchJob> hj(J);
   std::vector<BatchState> hs(J);
@@ -371,8 +380,8 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.ranges = nullptr;
     A.order = nullptr;
     A.cost = nullptr;
-    A.cull = nullptr;
-    A.ntmpl = 0;
+    A.cull = b->cull + cs[i];   // filled from the job's extents after the upload
+    A.ntmpl = s.num_templates;
     A.bin_shift = 0;
     d.alpha_ub = s.alpha_upper_bound;
     d.L.target = s.target;
@@ -437,6 +446,10 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     HIP_TRY(hipMemcpyAsync(b->dstate, stg + bj, bst, hipMemcpyHostToDevice, st));
   }
   HIP_TRY(hipMemcpyAsync(b->dstarts, starts, bss, hipMemcpyHostToDevice, st));
+  if (J) {
+    hipLaunchKernelGGL(k_batch_cull, dim3(J), dim3(64), 0, st, (const BatchJob*)b->djobs);
+    LAUNCH_CHECK("k_batch_cull");
+  }
   HIP_TRY(hipEventRecord(b->upload_done, st));
   b->upload_pending = true;
   b->J = J;

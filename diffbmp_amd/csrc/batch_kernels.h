@@ -66,6 +66,21 @@ __device__ __forceinline__ int find_job(const int* __restrict__ start, int J, in
   return lo;
 }
 
+// Content boxes of job j's templates for the raster cull, from its extents
+// (texels > 1e-6 mapped to u/v; empty templates (-1,1,-1,1)) widened by one
+// texel on each side, as k_template_extents<_, true> builds them from the
+// texels themselves.  One workgroup per job.
+static __global__ __launch_bounds__(64) void k_batch_cull(const BatchJob* __restrict__ jobs) {
+  const BatchJob& d = jobs[blockIdx.x];
+  float4* out = const_cast<float4*>(d.A.cull);
+  const float du = 2.f / (float)(d.A.Wt - 1), dv = 2.f / (float)(d.A.Ht - 1);
+  for (int t = threadIdx.x; t < d.T; t += 64) {
+    const float* e = d.ext + 4 * t;
+    const float u0 = e[0] - du, u1 = e[1] + du, v0 = e[2] - dv, v1 = e[3] + dv;
+    out[t] = make_float4(0.5f * (u0 + u1), 0.5f * (v0 + v1), 0.5f * (u1 - u0), 0.5f * (v1 - v0));
+  }
+}
+
 template <int DT>
 static __global__ __launch_bounds__(256) void k_batch_prep(const BatchJob* __restrict__ jobs,
                                                     const int* __restrict__ prim_start, int J,
