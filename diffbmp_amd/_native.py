@@ -255,6 +255,10 @@ class Handle:
         self.bin_factor = 4 if self.ts == 16 else 1
         self._pending = None          # (event, pinned count) of the last async binning
         self._lazy_calls = 0
+        # bumped by every call that recomputes the primitive records (a binning,
+        # or a raster call without reuse_records): a caller that noted it after
+        # preparing its parameters may pass reuse_records while it is unchanged
+        self.records_gen = 0
         self._tmpl_cache = None       # (key, converted templates, extents, version)
         self._tmpl_gen = 0
 
@@ -324,6 +328,7 @@ class Handle:
         ``factor`` > 1 bins per super-tile of factor x factor tiles and writes
         each primitive's tile range into ``ranges`` ((N,4) int32)."""
         T, Ht, Wt = tmpl_shape
+        self.records_gen += 1
         if factor > 1:
             check(self._L.dbmp_bin_tiles(self._h, ctypes.byref(prims), _ptr(ext), int(T), int(Ht),
                                          int(Wt), int(factor), _ptr(tile_offsets),
@@ -417,6 +422,8 @@ class Handle:
     def forward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T: float, out_color, out_alpha,
                 out_T, out_last) -> None:
         L = lists.c()
+        if not prims.reuse_records:
+            self.records_gen += 1
         check(self._L.dbmp_forward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(L),
                                    float(eps_T), _ptr(out_color), _ptr(out_alpha), _ptr(out_T),
                                    _ptr(out_last), _stream(out_color.device)))
@@ -424,6 +431,8 @@ class Handle:
     def backward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T, grad_color, grad_alpha,
                  out_color, out_T, out_last, g_means, g_r, g_th, g_v, g_c) -> None:
         L = lists.c()
+        if not prims.reuse_records:
+            self.records_gen += 1
         check(self._L.dbmp_backward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(L),
                                     float(eps_T), _ptr(grad_color), _ptr(grad_alpha), _ptr(out_color),
                                     _ptr(out_T), _ptr(out_last), _ptr(g_means), _ptr(g_r), _ptr(g_th),
@@ -434,6 +443,8 @@ class Handle:
                      bg_value: float = 1.0) -> None:
         """Background composite + MSE + backward in one raster pass."""
         L = lists.c()
+        if not prims.reuse_records:
+            self.records_gen += 1
         check(self._L.dbmp_backward_mse(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(L),
                                         float(eps_T), _ptr(target), _ptr(bg), float(bg_value),
                                         _ptr(loss_accum), _ptr(out_color), _ptr(out_T), _ptr(out_last),
@@ -446,6 +457,8 @@ class Handle:
         """backward_mse that adds the gradients into ``grad_rows`` ((N,8) fp32:
         x, y, r, theta, opacity logit, 3 colour logits)."""
         L = lists.c()
+        if not prims.reuse_records:
+            self.records_gen += 1
         check(self._L.dbmp_backward_mse_rows(self._h, ctypes.byref(prims), ctypes.byref(bm),
                                              ctypes.byref(L), float(eps_T), _ptr(target), _ptr(bg),
                                              float(bg_value), _ptr(loss_accum), _ptr(out_color),

@@ -136,19 +136,30 @@ class _Run:
         self.eps_T = float(eps_T)
         self.blend_grad_exact = bool(getattr(handle, "blend_grad_exact", False))
         self.handle = handle
+        self.gen = None   # handle.records_gen while the handle holds this call's records
+        self._prims = self._bitmaps = None
         if mapping is not None and mapping.numel() > 0:
             self.lists = _native.Lists(*unpack_mapping(mapping, handle.tiles))
         else:
             self.lists = handle.bin_lazy(self.prims(), self.ext, tuple(self.templates.shape),
                                          handle.bin_factor)
+            self.gen = handle.records_gen
 
     def prims(self):
-        return Handle.prims(self.means2D, self.radii, self.rotations, self.opacities, self.colors,
-                            self.sel)
+        # validated and packed once per call; the records the binning (or this
+        # call's forward) prepared are reused while no other call on the handle
+        # has replaced them
+        if self._prims is None:
+            self._prims = Handle.prims(self.means2D, self.radii, self.rotations, self.opacities,
+                                       self.colors, self.sel)
+        self._prims.reuse_records = int(self.gen is not None and self.gen == self.handle.records_gen)
+        return self._prims
 
     def bitmaps(self):
-        return Handle.bitmaps(self.templates, self.colors_orig, self.co_mode, self.c_blend,
-                              self.tmpl_version, self.blend_grad_exact)
+        if self._bitmaps is None:
+            self._bitmaps = Handle.bitmaps(self.templates, self.colors_orig, self.co_mode,
+                                           self.c_blend, self.tmpl_version, self.blend_grad_exact)
+        return self._bitmaps
 
     def forward(self):
         dev, H, W = self.means2D.device, self.handle.H, self.handle.W
@@ -158,6 +169,7 @@ class _Run:
         out_last = torch.empty((H, W), dtype=torch.int32, device=dev)
         self.handle.forward(self.prims(), self.bitmaps(), self.lists, self.eps_T,
                             out_color, out_alpha, out_T, out_last)
+        self.gen = self.handle.records_gen
         return out_color, out_alpha, out_T, out_last
 
     def backward(self, grad_color, grad_alpha, out_color, out_T, out_last):
