@@ -99,3 +99,35 @@ def test_errors_raise(dev):
              torch.from_numpy(sc.bmp_sel), 0.0, torch.zeros(7))
     with pytest.raises(Exception):
         TR.TileRasterizer(32, 32, 24)   # tile size must be a multiple of 16
+
+
+def test_interleaved_calls_recompute_replaced_records(dev):
+    """Two forwards on one rasterizer before either backward: the second
+    replaces the handle's primitive records, so the first call's backward must
+    prepare its own again.  Its gradients equal those of a lone forward and
+    backward of the same parameters."""
+    sc = S.make_scene(0, with_target=False)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rast = TR.TileRasterizer(sc.H, sc.W, 16, 0.0, 1.0, 151, sc.N)
+    lr = torch.zeros(7, device=dev)
+    args = (torch.empty(0, device=dev), t(sc.templates), t(sc.bmp_sel).long(), 0.0, lr)
+    target = torch.rand((sc.H, sc.W, 3), device=dev, generator=torch.Generator(dev).manual_seed(1))
+
+    def loss(color, alpha):
+        return torch.nn.functional.mse_loss(color + (1 - alpha.unsqueeze(-1)), target)
+
+    A = _leaves(sc, dev)
+    loss(*rast(*A, *args)).backward()
+    lone = [p.grad.clone() for p in A]
+    A2 = _leaves(sc, dev)
+    B = _leaves(sc, dev)
+    with torch.no_grad():
+        B[0].add_(3.0)          # different parameters: different records
+        B[2].add_(0.5)
+    la = loss(*rast(*A2, *args))
+    lb = loss(*rast(*B, *args))
+    la.backward()
+    lb.backward()
+    for g, p in zip(lone, A2):
+        err = ((p.grad - g).norm() / g.norm().clamp_min(1e-30)).item()
+        assert err <= 1e-5, err
