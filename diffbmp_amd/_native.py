@@ -325,11 +325,13 @@ class Handle:
                   pair_count: torch.Tensor, factor: int = 1,
                   ranges: torch.Tensor | None = None) -> None:
         """Launch binning into caller buffers; no host synchronisation.
-        ``factor`` > 1 bins per super-tile of factor x factor tiles and writes
-        each primitive's tile range into ``ranges`` ((N,4) int32)."""
+        ``factor`` > 1 bins per super-tile of factor x factor tiles.  Each
+        primitive's tile range goes into ``ranges`` ((N,4) int32; required for
+        ``factor`` > 1, optional for per-tile lists): after a pair overflow the
+        raster kernels walk every primitive and use it only inside that range."""
         T, Ht, Wt = tmpl_shape
         self.records_gen += 1
-        if factor > 1:
+        if factor > 1 or ranges is not None:
             check(self._L.dbmp_bin_tiles(self._h, ctypes.byref(prims), _ptr(ext), int(T), int(Ht),
                                          int(Wt), int(factor), _ptr(tile_offsets),
                                          _ptr(tile_indices), int(capacity), _ptr(pair_count),
@@ -339,9 +341,8 @@ class Handle:
                                _ptr(tile_offsets), _ptr(tile_indices), int(capacity),
                                _ptr(pair_count), _stream(ext.device)))
 
-    def _ranges(self, prims: Prims, dev, factor: int):
-        if factor <= 1:
-            return None
+    @staticmethod
+    def _ranges(prims: Prims, dev):
         return torch.empty((max(prims.num_primitives, 1), 4), dtype=torch.int32, device=dev)
 
     def bin(self, prims: Prims, ext: torch.Tensor, tmpl_shape, factor: int = 1):
@@ -354,7 +355,7 @@ class Handle:
         cap = max(self.pair_capacity, 4 * N + 1024)
         offs = torch.empty(self.list_count(factor) + 1, dtype=torch.int32, device=dev)
         cnt = torch.empty(1, dtype=torch.int64, device=dev)
-        rg = self._ranges(prims, dev, factor)
+        rg = self._ranges(prims, dev) if factor > 1 else None
         for _ in range(2):
             idx = torch.empty(cap, dtype=torch.int32, device=dev)
             self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
@@ -392,7 +393,9 @@ class Handle:
         every ``READBACK_EVERY`` calls (each read-back is a device-to-host copy
         on the stream); the first call, with no history, reads the count once.
         If a call overflows, the raster kernels walk every primitive for that
-        call (exact, slower) until a read-back grows the buffer."""
+        call, each within the tile range the binning gave it (the lists'
+        membership, so the result is the same, only slower), until a read-back
+        grows the buffer."""
         dev = ext.device
         N = prims.num_primitives
         self._lazy_calls += 1
@@ -409,7 +412,7 @@ class Handle:
         offs = torch.empty(self.list_count(factor) + 1, dtype=torch.int32, device=dev)
         idx = torch.empty(cap, dtype=torch.int32, device=dev)
         cnt = torch.empty(1, dtype=torch.int64, device=dev)
-        rg = self._ranges(prims, dev, factor)
+        rg = self._ranges(prims, dev)
         self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
         if self._pending is None and self._lazy_calls % self.READBACK_EVERY == 0:
             host = torch.empty(1, dtype=torch.int64, pin_memory=True)

@@ -377,7 +377,9 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.tiles = ts[i + 1] - ts[i];
     A.subs = 1;
     A.eps = s.eps_T;
-    A.ranges = nullptr;
+    // the job's binning ranges: after a pair overflow every pixel walks its
+    // job's primitives, each kept to the tiles of its range
+    A.ranges = b->ranges + ps[i];
     A.order = nullptr;
     A.cost = nullptr;
     A.cull = b->cull + cs[i];   // filled from the job's extents after the upload

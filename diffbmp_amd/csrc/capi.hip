@@ -288,6 +288,11 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.subs = h->subs;
   A.eps = eps;
   const int f = L->bin_factor;
+  if (f <= 1 && L->pair_count != nullptr) {
+    // lists from dbmp_bin: after an overflow every primitive is walked, and
+    // the binning's ranges keep each one to the tiles whose lists hold it
+    A.ranges = L->ranges != nullptr ? (const int4*)L->ranges : h->ranges;
+  }
   if (f > 1) {   // super-tile lists (checked by check_lists)
     int s = 0;
     while ((1 << s) < f) ++s;

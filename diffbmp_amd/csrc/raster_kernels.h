@@ -40,9 +40,12 @@ struct RasterArgs {
   long long pcap;           // capacity the lists were built with
   const void* quads;
   const void* cmap;      // colour maps, or null: no colour-map blend
-  const int4* ranges;    // lists per super-tile of 16 << bin_shift px: each primitive's
-                         // 16-px tile range (tx0, tx1, ty0, ty1); an entry is used in
-                         // a 16x16 block only inside that range.  Null: per-tile lists
+  const int4* ranges;    // each primitive's binning tile range (tx0, tx1, ty0, ty1), in
+                         // tiles of ts >> bin_shift px (16 for super-tile lists, ts for
+                         // per-tile lists); an entry is used in a 16x16 block only
+                         // inside that range.  Super-tile lists always read it; per-tile
+                         // lists only when binning overflowed and every primitive is
+                         // walked.  Null: caller-supplied, complete per-tile lists
   int cmap_mode;         // 1: cmap = (N,Ht,Wt,3) by primitive, quads = alpha quads;
                          // 2: quads = RGBA texel records (RecT) of the per-template
                          //    maps (cmap = quads when blending)
@@ -149,6 +152,7 @@ constexpr int kThreads = 256; // 4 waves per 16x16 block
 
 struct BlockGeom {
   int tile, bx, by, px, py, w, lane;
+  int rtx, rty;   // the block's tile in the units of A.ranges
 };
 
 // Geometry of workgroup b (already remapped) of a canvas.
@@ -160,6 +164,9 @@ __device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A, int b) {
   const int tx = g.tile % A.tilesX, ty = g.tile / A.tilesX;
   g.bx = tx * A.ts + (sub % spr) * kSub;
   g.by = ty * A.ts + (sub / spr) * kSub;
+  // ranges count 16-px tiles for super-tile lists, list tiles otherwise
+  g.rtx = A.bin_shift > 0 ? g.bx >> 4 : tx;
+  g.rty = A.bin_shift > 0 ? g.by >> 4 : ty;
   // wave-uniform by construction; readfirstlane lets the compiler keep the
   // cull tests and loop counters in scalar registers
   g.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -264,20 +271,25 @@ __device__ __forceinline__ Stage stage_load(const RasterArgs& A, bool all, int k
   Stage st;
   st.v = fetch_rec_part(A, all, k, end, part, st.n);
   st.rg = make_int4(0, 0x7fffffff, 0, 0x7fffffff);
-  if (A.ranges != nullptr && part == 0 && st.n >= 0) st.rg = A.ranges[st.n];
+  // complete per-tile lists hold an entry only in tiles of its range, so the
+  // range is read where it decides: super-tile lists, and the walk over every
+  // primitive after a binning overflow
+  if (A.ranges != nullptr && part == 0 && st.n >= 0 && (all || A.bin_shift > 0))
+    st.rg = A.ranges[st.n];
   return st;
 }
 __device__ __forceinline__ void stage_store(const Stage& st, PrimRec (*srec)[kBatch], int (*sid)[kBatch],
-                                            int (*smask)[kBatch], int buf, int ri, int part, int bx,
-                                            int by) {
+                                            int (*smask)[kBatch], int buf, int ri, int part,
+                                            const BlockGeom& G) {
   reinterpret_cast<float4*>(&srec[buf][ri])[part] = st.v;
   if (part == 0) {
     sid[buf][ri] = st.n;
-    // the entry belongs to this block's 16-px tile's list only inside the
-    // primitive's binning range (the reference's per-tile lists)
-    const int tx = bx >> 4, ty = by >> 4;
+    // the entry belongs to this block's tile's list only inside the
+    // primitive's binning range (the reference's per-tile lists,
+    // vector_renderer.py:401-444: a texel fringe outside it is not drawn)
+    const int tx = G.rtx, ty = G.rty;
     const bool in = (tx >= st.rg.x) & (tx <= st.rg.y) & (ty >= st.rg.z) & (ty <= st.rg.w);
-    smask[buf][ri] = in ? wave_mask(st.v, bx, by) : 0;
+    smask[buf][ri] = in ? wave_mask(st.v, G.bx, G.by) : 0;
   }
 }
 
@@ -372,7 +384,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   const int wbit = 1 << G.w;
   const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
   const float ccy = uniform_f((float)(G.by + 8 * (G.w >> 1)) + 3.5f);
-  stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
+  stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G);
   __syncthreads();
   int buf = 0;
   for (int base = start; base < end; base += kBatch, buf ^= 1) {
@@ -439,7 +451,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         }
       }
     }
-    if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G.bx, G.by);
+    if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G);
     // one barrier: it publishes the staged batch and each wave's liveness
     // (one flag per wave and buffer parity, so a flag is rewritten only after
     // every wave has read it; __syncthreads_or takes three barriers)
@@ -558,7 +570,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   const int wbit = 1 << G.w;
   const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
   const float ccy = uniform_f((float)(G.by + 8 * (G.w >> 1)) + 3.5f);
-  stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G.bx, G.by);
+  stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G);
   __syncthreads();
   const int stop = start + min(skmax, end - start);
   int buf = 0;
@@ -661,7 +673,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const float tot = wave_reduce_scatter8(acc, which);
       if ((G.lane & 7) == 0) sred[G.w][j][which] = tot;
     }
-    if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G.bx, G.by);
+    if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G);
     __syncthreads();
     // one atomic per (tile, entry, value): sum the four waves' partials and
     // apply the per-primitive factor of value q

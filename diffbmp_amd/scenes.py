@@ -35,7 +35,7 @@ import numpy as np
 
 __all__ = [
     "Scene", "make_scene", "shape_bitmap", "smooth_field", "cyclic_sel",
-    "CONFIGS", "plain_disk", "margin_bitmaps",
+    "CONFIGS", "plain_disk", "margin_bitmaps", "hard_square",
 ]
 
 
@@ -154,6 +154,20 @@ def shape_bitmap(kind: str, size: int, rng: np.random.Generator | None = None,
 
 
 SHAPES = ("disk", "star", "ring", "noise")
+
+
+def hard_square(size: int = 64, inset: int | None = None) -> np.ndarray:
+    """Binary centre square: 1 on texels ``[inset, size-1-inset]`` in both
+    axes, 0 elsewhere (``inset`` defaults to ``size // 4``).  Its content does
+    not reach the bitmap's edge and its bilinear fringe is hard: a sample up to
+    one texel outside the content rows and columns still has a mask > 1e-6,
+    so a primitive draws a little beyond the box its template extents give
+    (the binning box).  The reference's own bitmaps are hard-edged like this
+    (e.g. ``assets/primitives/circle.png``)."""
+    k = size // 4 if inset is None else int(inset)
+    out = np.zeros((size, size), np.float32)
+    out[k:size - k, k:size - k] = 1.0
+    return out
 
 
 def smooth_field(H: int, W: int, rng: np.random.Generator, terms: int = 32) -> np.ndarray:

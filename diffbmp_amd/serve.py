@@ -52,6 +52,7 @@ class JobSpec:
     arrival: float = 0.0              # seconds from the start of the run
     name: str = ""
     template_size: int = 256
+    library: str = "stress"           # template_library kind
 
 
 def job_bytes(spec: JobSpec, store_dtype=torch.float16) -> int:
@@ -95,14 +96,23 @@ class HbmBudget:
 _LIBRARY: dict = {}
 
 
-def template_library(size: int = 256, count: int = 16):
-    """The stress workload's RGBA bitmaps: alpha templates (count, size, size)
-    and per-template colour maps (count, size, size, 3), fp32 numpy."""
-    key = (size, count)
+def template_library(size: int = 256, count: int = 16, kind: str = "stress"):
+    """Alpha templates (count, size, size) and per-template colour maps
+    (count, size, size, 3), fp32 numpy.  ``kind`` "stress": the stress
+    workload's RGBA bitmaps; "hard": binary centre squares of inset
+    size/8 .. 3 size/8 (hard-edged, content short of the bitmap edge) with the
+    stress workload's colour maps."""
+    key = (size, count, kind)
     if key not in _LIBRARY:
         from . import scenes as S
         sc = S.make_scene(3, H=64, W=64, N=1, tsz=size, with_target=False)
-        _LIBRARY[key] = (sc.templates[:count].copy(), sc.colors_orig_tmpl[:count].copy())
+        tm, cm = sc.templates[:count].copy(), sc.colors_orig_tmpl[:count].copy()
+        if kind == "hard":
+            tm = np.stack([S.hard_square(size, size // 8 + (i * size) // (4 * max(count, 1)))
+                           for i in range(count)])
+        elif kind != "stress":
+            raise ValueError(f"unknown template library {kind!r}")
+        _LIBRARY[key] = (tm, cm)
     return _LIBRARY[key]
 
 
@@ -145,7 +155,7 @@ class FitState:
         self.spec = spec
         H, W, N = spec.height, spec.width, spec.num_primitives
         rng = np.random.default_rng(spec.seed)
-        lib_t, lib_c = template_library(spec.template_size)
+        lib_t, lib_c = template_library(spec.template_size, kind=spec.library)
         ids = list(spec.template_ids)
         T = len(ids)
         Ht, Wt = lib_t.shape[1:]

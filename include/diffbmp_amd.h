@@ -74,11 +74,13 @@ typedef struct {
 } dbmp_bitmaps;
 
 /* CSR tile lists: tile_offsets (tiles+1), tile_indices (P).  When the lists
- * come from dbmp_bin, pass its device pair count and the capacity it was
- * given: if the count exceeded the capacity (the lists were not written),
- * the raster kernels walk every primitive 0..N-1 for every tile instead, which
- * gives the same image and gradients, only slower.  pair_count = NULL means
- * the lists are complete (e.g. a caller-supplied mapping). */
+ * come from dbmp_bin / dbmp_bin_tiles, pass its device pair count and the
+ * capacity it was given: if the count exceeded the capacity (the lists were
+ * not written), the raster kernels walk every primitive 0..N-1 for every tile
+ * instead and use a primitive in a tile only inside its binning tile range --
+ * the membership the lists would have held -- so image and gradients are the
+ * same, only slower.  pair_count = NULL means the lists are complete (e.g. a
+ * caller-supplied mapping) and ranges are not needed. */
 typedef struct {
   const int32_t* tile_offsets;
   const int32_t* tile_indices;
@@ -86,7 +88,10 @@ typedef struct {
   int64_t capacity;
   int32_t bin_factor;       /* 0 or 1: one list per tile; f > 1: lists from
                                dbmp_bin_tiles, one per super-tile of f x f tiles */
-  const int32_t* ranges;    /* f > 1: (N,4) tile ranges written by dbmp_bin_tiles */
+  const int32_t* ranges;    /* (N,4) tile ranges written by the binning that made the
+                               lists (dbmp_bin_tiles' ranges_out); required for f > 1.
+                               For f <= 1 with pair_count set, NULL means the ranges
+                               the handle's latest dbmp_bin wrote */
 } dbmp_tile_lists;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -132,9 +137,11 @@ int dbmp_bin(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int 
 /* Binning into coarser lists for the raster calls of this handle: one list
  * per super-tile of bin_factor x bin_factor tiles (bin_factor 2, 4 or 8; tile
  * size 16), ascending primitive id, plus each primitive's tile range (N,4)
- * int32 in ranges_out.  The raster kernels use an entry in a tile only inside
- * that range, so forward and backward are exactly those of dbmp_bin's per-tile
- * lists, with bin_factor^2 fewer pairs to sort.  tile_offsets holds
+ * int32 in ranges_out (bin_factor 1: the per-tile lists of dbmp_bin, with
+ * ranges_out optional; NULL keeps them in the handle).  The raster kernels
+ * use an entry in a tile only inside that range, so forward and backward are
+ * exactly those of dbmp_bin's per-tile lists, with bin_factor^2 fewer pairs
+ * to sort.  tile_offsets holds
  * ceil(tilesX/f)*ceil(tilesY/f)+1 entries.  Pass bin_factor and ranges_out on
  * in dbmp_tile_lists. */
 int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext, int num_templates,

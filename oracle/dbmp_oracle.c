@@ -147,6 +147,14 @@ static void color_map(const float *cm, int Ht, int Wt, REAL cx, REAL cy, REAL co
     }
 }
 
+/* The colour-map sample above at texel coordinates (cx, cy) of one (Ht,Wt,3)
+ * map, with its derivative; exported so tests pin it with closed forms. */
+int oracle_color_map_sample(const float *cm, int Ht, int Wt, double cx, double cy, REAL *co,
+                            REAL *dcx, REAL *dcy) {
+    color_map(cm, Ht, Wt, (REAL)cx, (REAL)cy, co, dcx, dcy);
+    return 0;
+}
+
 static int sample_prim(const dbmp_oracle_scene *s, int n, int px, int py, sample_t *o) {
     const int Ht = s->Ht, Wt = s->Wt;
     const int ti = s->bmp_sel[n];

@@ -131,3 +131,51 @@ def test_interleaved_calls_recompute_replaced_records(dev):
     for g, p in zip(lone, A2):
         err = ((p.grad - g).norm() / g.norm().clamp_min(1e-30)).item()
         assert err <= 1e-5, err
+
+
+@pytest.mark.parametrize("ts", [32, 16])
+def test_hard_template_overflow_through_autograd(dev, ts):
+    """TileRasterizer at the reference's default tile size 32 (per-tile
+    lists) and at 16 (super-tile lists), hard-edged template, lazy binning
+    forced to overflow its pair buffer, and a second forward of other
+    parameters before the first backward: every primitive is walked within
+    its own call's binning ranges, so image and gradients equal the oracle's
+    on the binned lists."""
+    sc = S.make_scene(0, with_target=False)
+    sc.templates = S.hard_square(sc.templates.shape[1])[None]
+    sc.tile_size = ts
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rast = TR.TileRasterizer(sc.H, sc.W, ts, 0.0, 1.0, 151, sc.N)
+    rast.eps_T = 0.0
+    lr = torch.zeros(7, device=dev)
+    args = (torch.empty(0, device=dev), t(sc.templates), t(sc.bmp_sel), 0.0, lr)
+    rast(*_leaves(sc, dev), *args)          # sizes the pair buffer
+    torch.cuda.synchronize()
+    rast.handle.pair_capacity = 64          # the next calls overflow
+    A = _leaves(sc, dev)
+    cA, aA = rast(*A, *args)
+    B = _leaves(sc, dev)
+    with torch.no_grad():
+        B[0].add_(5.0)
+        B[2].add_(0.7)
+    cB, aB = rast(*B, *args)
+    assert int(rast.handle.pair_capacity) == 64
+    gC, gA = (torch.from_numpy(a).to(dev) for a in
+              (np.random.default_rng(2).standard_normal((sc.H, sc.W, 3)).astype(np.float32),
+               np.random.default_rng(3).standard_normal((sc.H, sc.W)).astype(np.float32)))
+    ((cA * gC).sum() + (aA * gA).sum()).backward()
+    ((cB * gC).sum() + (aB * gA).sum()).backward()
+    ext = O.template_extents_np(sc.templates)
+    offs, idx = O.bin_lists_c(sc.means2D, sc.radii, sc.rotations, sc.bmp_sel, ext, sc.H, sc.W, ts)
+    osc = O.scene_from(sc, offs, idx)
+    rc, ra, _ = O.forward(osc)
+    e = max(np.abs(cA.detach().cpu().numpy() - rc).max(), np.abs(aA.detach().cpu().numpy() - ra).max())
+    print(f"ts {ts}: image error {e:.2e}")
+    assert e <= 1e-5
+    ref = O.backward(osc, gC.cpu().numpy().astype(np.float64), gA.cpu().numpy().astype(np.float64))
+    got = dict(means2D=A[0].grad.cpu().numpy(), radii=A[1].grad.cpu().numpy(),
+               rotations=A[2].grad.cpu().numpy(), opacities=A[3].grad.cpu().numpy(),
+               colors=A[4].grad.cpu().numpy())
+    err = normwise(got, ref)
+    print("gradient error", err)
+    assert max(err.values()) <= 1e-4

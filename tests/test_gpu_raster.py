@@ -80,25 +80,47 @@ def test_scale_sampled_tiles(dev):
     assert max(err.values()) <= 1e-4
 
 
-def test_overflow_fallback_is_exact(dev):
+def _hard_scene(**kw):
+    """The plain workload with a hard-edged template (binary centre square):
+    its bilinear fringe reaches past the binned box, so only the binning's
+    tile ranges keep a walk over every primitive equal to the lists."""
+    sc = S.make_scene(0, with_target=False, **kw)
+    sc.templates = S.hard_square(sc.templates.shape[1])[None]
+    return sc
+
+
+@pytest.mark.parametrize("kind,ts,explicit", [("soft", 16, False), ("hard", 16, False),
+                                              ("hard", 16, True), ("hard", 32, False),
+                                              ("hard", 32, True)])
+def test_overflow_fallback_is_exact(dev, kind, ts, explicit):
     """Binning into a too-small buffer: the raster kernels walk every
-    primitive instead and produce the same image and gradients."""
+    primitive instead, each within its binning tile range, and produce the
+    same image and gradients (and the oracle's, for the binned lists).
+    ``explicit``: the lists carry their own ranges buffer; otherwise the
+    handle's, from its latest binning."""
     import torch
     from diffbmp_amd import _native
-    sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    sc = _hard_scene() if kind == "hard" else S.make_scene(0, with_target=False)
+    sc.tile_size = ts
+    run = GpuRun(sc, dev, eps_T=1e-6, ts=ts)
     col, alp = run.forward()
     gC, gA = rand_grads(sc.H, sc.W, seed=9)
     g = run.backward(gC, gA)
+    osc = O.scene_from(sc, *_oracle_lists(sc))
+    rc, ra, _ = O.forward(osc)
+    assert max(np.abs(col - rc).max(), np.abs(alp - ra).max()) <= 1e-5
+    assert max(normwise(g, O.backward(osc, gC.astype(np.float64), gA.astype(np.float64))).values()) <= 1e-4
     P = run.idx.numel()
     offs = torch.empty_like(run.offs)
     idx = torch.empty(P // 3, dtype=torch.int32, device=dev)
     cnt = torch.empty(1, dtype=torch.int64, device=dev)
-    run.h.bin_async(run.prims, run.ext, tuple(run.d["templates"].shape), P // 3, offs, idx, cnt)
+    rg = torch.empty((sc.N, 4), dtype=torch.int32, device=dev) if explicit else None
+    run.h.bin_async(run.prims, run.ext, tuple(run.d["templates"].shape), P // 3, offs, idx, cnt,
+                    1, rg)
     assert int(cnt.item()) == P
     assert int(offs.abs().sum().item()) == 0
     run.offs, run.idx = offs, idx
-    lists = _native.Lists(offs, idx, cnt, P // 3)
+    lists = _native.Lists(offs, idx, cnt, P // 3, 1, rg)
     H, W = sc.H, sc.W
     c2 = torch.empty((H, W, 3), device=dev); a2 = torch.empty((H, W), device=dev)
     T2 = torch.empty((H, W), device=dev); l2 = torch.empty((H, W), dtype=torch.int32, device=dev)
@@ -116,14 +138,17 @@ def test_overflow_fallback_is_exact(dev):
         assert np.linalg.norm(t.cpu().numpy() - ref) <= 1e-5 * max(np.linalg.norm(ref), 1e-30), k
 
 
-def test_lazy_binning_overflow_then_grows(dev):
-    """Lazy binning with too small a pair buffer: every call still gives the
-    exact image (the raster kernels walk all primitives), and the capacity
-    grows once an asynchronous read-back (one per READBACK_EVERY calls) has
-    seen the count."""
+@pytest.mark.parametrize("ts", [16, 32])
+def test_lazy_binning_overflow_then_grows(dev, ts):
+    """Lazy binning with too small a pair buffer (hard-edged template; super-
+    tile lists at tile size 16, per-tile lists at 32): every call still gives
+    the exact image (the raster kernels walk all primitives within their
+    binning ranges), and the capacity grows once an asynchronous read-back
+    (one per READBACK_EVERY calls) has seen the count."""
     from diffbmp_amd import _native
-    sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    sc = _hard_scene()
+    sc.tile_size = ts
+    run = GpuRun(sc, dev, eps_T=1e-6, ts=ts)
     col, alp = run.forward()
     h, shape = run.h, tuple(run.d["templates"].shape)
     P = h.bin(run.prims, run.ext, shape, h.bin_factor)[1].numel()   # pairs of the lists bin_lazy builds

@@ -116,9 +116,14 @@ def test_graph_replay_matches_eager(dev):
     assert eg.set_count == 2
 
 
-def test_pair_overflow_is_exact(dev):
+@pytest.mark.parametrize("library", ["stress", "hard"])
+def test_pair_overflow_is_exact(dev, library):
+    """A pair-capacity overflow walks every primitive of the job, each within
+    its binning tile range, so even a hard-edged template (whose bilinear
+    fringe reaches past its binned box) renders exactly as with the lists."""
+    jb = spec("B", 160, 208, 900, 4, seed=12, library=library)
     small, big = engine(pairs_per_primitive=0.05), engine()
-    (x,), (y,) = states(small, [B]), states(big, [B])
+    (x,), (y,) = states(small, [jb]), states(big, [jb])
     small.step()
     big.step()
     torch.cuda.synchronize()

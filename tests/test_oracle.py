@@ -11,6 +11,13 @@ would fail:
 * forward            -- closed forms: one layer, two layers (OVER), bilinear
                         at integer texel coordinates, push-box edge, mask gate,
                         invalid template, list order;
+* colour-map sample  -- closed forms: constant map, integer texel, linear
+                        ramp (value and slopes) at fractional coordinates,
+                        clamp to the edge; through the forward with a rotated
+                        primitive;
+* list membership    -- walking every primitive within its binning range
+                        equals the binned lists; without the ranges a
+                        hard-edged template's fringe changes the image;
 * backward           -- central finite differences (h = 1e-3, fp64) on
                         random scalars, with binning frozen; closed form for
                         the colour-logit gradient;
@@ -355,3 +362,120 @@ def test_fp32_port_agrees_with_fp64_oracle():
     g32 = O.backward(osc, gC.astype(f32), gA.astype(f32), "f32")
     for k in g64:
         assert np.linalg.norm(g64[k] - g32[k]) <= 1e-4 * np.linalg.norm(g64[k])
+
+
+# ---------------------------------------------------------------- colour-map sample
+def _cmap_sample(cm, cx, cy):
+    import ctypes
+    L = O.lib("f64")
+    Ht, Wt, _ = cm.shape
+    cm = np.ascontiguousarray(cm, f32)
+    co, dx, dy = np.zeros(3), np.zeros(3), np.zeros(3)
+    L.oracle_color_map_sample(O._p(cm), Ht, Wt, ctypes.c_double(cx), ctypes.c_double(cy),
+                              O._p(co), O._p(dx), O._p(dy))
+    return co, dx, dy
+
+
+def _ramp_map(Ht, Wt):
+    """c(x, y, ch) = a_ch x + b_ch y + d_ch with different slopes along x and y
+    (and per channel), so a swapped x/y corner or a transposed map changes
+    the value off the diagonal."""
+    a = np.array([0.05, -0.03, 0.011])
+    b = np.array([0.02, 0.07, -0.013])
+    d = np.array([0.1, 0.5, 0.9])
+    yy, xx = np.mgrid[0:Ht, 0:Wt]
+    return (a * xx[..., None] + b * yy[..., None] + d).astype(f32), a, b, d
+
+
+def test_colour_map_sample_closed_forms():
+    """sample_color_map (tile_forward.cu:17-89): clamp-to-edge bilinear RGB at
+    texel coordinates.  Pinned by (i) a constant map returns the constant,
+    (ii) integer coordinates return that texel, (iii) bilinear interpolation
+    reproduces a linear ramp exactly at fractional coordinates, with the
+    ramp's slopes as derivative (pins the x/y orientation of the four
+    corners), (iv) coordinates outside [0, W-1] x [0, H-1] take the edge
+    texel's value (clamp, not zero padding)."""
+    Ht, Wt = 5, 7
+    const = np.zeros((Ht, Wt, 3), f32) + np.array([0.25, 0.5, 0.75], f32)
+    for cx, cy in ((0.0, 0.0), (3.3, 1.7), (6.0, 4.0), (-2.0, 9.0)):
+        co, _, _ = _cmap_sample(const, cx, cy)
+        assert np.array_equal(co, [0.25, 0.5, 0.75])
+    cm, a, b, d = _ramp_map(Ht, Wt)
+    for y in range(Ht):
+        for x in range(Wt):
+            co, _, _ = _cmap_sample(cm, float(x), float(y))
+            assert np.array_equal(co, cm[y, x].astype(np.float64)), (x, y)
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        cx, cy = rng.uniform(0, Wt - 1), rng.uniform(0, Ht - 1)
+        co, dcx, dcy = _cmap_sample(cm, cx, cy)
+        assert np.abs(co - (a * cx + b * cy + d)).max() < 1e-6
+        assert np.abs(dcx - a).max() < 1e-6 and np.abs(dcy - b).max() < 1e-6
+    for cx, cy, ex, ey in ((-3.5, 2.25, 0, 2.25), (Wt + 4.0, 1.5, Wt - 1, 1.5),
+                           (2.5, -1.0, 2.5, 0), (4.25, Ht + 0.5, 4.25, Ht - 1),
+                           (-1.0, -1.0, 0, 0), (Wt + 1.0, Ht + 1.0, Wt - 1, Ht - 1)):
+        co, _, _ = _cmap_sample(cm, cx, cy)
+        assert np.abs(co - (a * ex + b * ey + d)).max() < 1e-6, (cx, cy)
+
+
+def test_forward_colour_map_closed_form_rotated():
+    """One primitive with an all-ones template (mask 1 inside the texture
+    bounds), theta = pi/2 and a linear ramp colour map blended with k: the
+    pixel colour is alpha ((1-k) sig(c) + k ramp(tx, ty)) with, for theta =
+    pi/2, tx = (dy/r + 1)(Wt-1)/2 and ty = (-dx/r + 1)(Ht-1)/2 -- the map
+    turns with the primitive.  A mirrored rotation, a transposed map or an
+    unblended colour fails."""
+    Ht, Wt = 9, 9
+    cm, a, b, d = _ramp_map(Ht, Wt)
+    tm = np.ones((1, Ht, Wt), f32)
+    k = 0.6
+    r = 8.0
+    cl = [0.3, -0.4, 1.1]
+    sc = _scene(48, 48, [[20, 22]], [r], [math.pi / 2], [1.5], [cl], tm,
+                colors_orig=cm[None], c_blend=k)
+    col, al, _ = O.forward(sc)
+    alpha = sig(1.5)
+    checked = 0
+    for py in range(16, 29):
+        for px in range(14, 27):
+            dx, dy = px - 20.0, py - 22.0
+            if abs(dx) > r - 0.5 or abs(dy) > r - 0.5:
+                continue
+            tx = (dy / r + 1.0) * (Wt - 1) / 2.0
+            ty = (-dx / r + 1.0) * (Ht - 1) / 2.0
+            ramp = a * tx + b * ty + d
+            want = alpha * ((1 - k) * np.array([sig(c) for c in cl]) + k * ramp)
+            assert np.abs(col[py, px] - want).max() < 2e-6, (px, py)
+            checked += 1
+    assert checked >= 100
+
+
+# ---------------------------------------------------------------- walk-all fallback
+def test_walk_all_needs_binning_ranges_for_hard_edges():
+    """The tile lists decide membership (vector_renderer.py:401-444): a
+    hard-edged template's bilinear fringe reaches up to one texel past the
+    box its extents give, and the reference does not draw that fringe in a
+    tile whose list lacks the primitive.  Walking every primitive in every
+    tile therefore changes the image of a hard-edged template (so the GPU
+    overflow tests, which use one, fail a fallback that ignores the binning
+    ranges), while the soft disk is unchanged; walking every primitive
+    within its binning tile range gives exactly the binned lists."""
+    for tmpl, differs in ((S.hard_square(64)[None], True), (None, False)):
+        sc = S.make_scene(0, with_target=False)
+        if tmpl is not None:
+            sc.templates = tmpl
+        offs, idx = O.bin_scene(sc)
+        tiles = len(offs) - 1
+        col, al, _ = O.forward(O.scene_from(sc, offs, idx))
+        all_offs = (np.arange(tiles + 1) * sc.N).astype(np.int32)
+        all_idx = np.tile(np.arange(sc.N, dtype=np.int32), tiles)
+        col2, al2, _ = O.forward(O.scene_from(sc, all_offs, all_idx))
+        e = max(np.abs(col - col2).max(), np.abs(al - al2).max())
+        print("hard" if differs else "soft", "walk-all vs lists:", e)
+        assert (e > 0.05) if differs else (e < 1e-9)
+        # every primitive, kept to the tiles of its binning range
+        bb = O.bin_bboxes_np(sc.means2D, sc.radii, sc.rotations, sc.bmp_sel,
+                             O.template_extents_np(sc.templates))
+        dense = O.bin_dense_np(bb, sc.H, sc.W, sc.tile_size)
+        r_offs, r_idx = O.lists_from_dense(dense)
+        assert np.array_equal(r_offs, offs) and np.array_equal(r_idx, idx)
