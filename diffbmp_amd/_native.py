@@ -130,6 +130,11 @@ EXPORTS = {
                                     ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
                                     ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_forward_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                            ctypes.POINTER(Prims), ctypes.POINTER(Bitmaps),
+                                            ctypes.POINTER(TileLists), ctypes.c_float,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
     "dbmp_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims),
                                      ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
                                      ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,

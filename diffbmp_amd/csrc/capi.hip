@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -41,6 +42,38 @@ __global__ void k_unpack_grads(const float* __restrict__ g8, int N, float* __res
   gc[3 * n] = b.y;
   gc[3 * n + 1] = b.z;
   gc[3 * n + 2] = b.w;
+}
+
+// Scratch of dbmp_forward_noalloc: primitive records | template content
+// boxes | texel quads or RGBA records, at fixed offsets.  One copy per GPU,
+// held until the process ends; every call on a GPU shares it.
+constexpr int kNoAllocMaxPrims = 65536;
+constexpr int kNoAllocMaxTemplates = 4096;
+constexpr size_t kNoAllocRecBytes = (size_t)kNoAllocMaxPrims * sizeof(dbmp::PrimRec);   // 4 MiB
+constexpr size_t kNoAllocCullBytes = (size_t)kNoAllocMaxTemplates * sizeof(float4);     // 64 KiB
+constexpr size_t kNoAllocQuadBytes = 16u << 20;                                         // 16 MiB
+constexpr size_t kNoAllocBytes = kNoAllocRecBytes + kNoAllocCullBytes + kNoAllocQuadBytes;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (the host-side symbol is
+// only a stand-in): looked up once per device with hipGetSymbolAddress.
+int scratch_address(unsigned char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(DBMP_E_ARG, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(DBMP_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
 }
 
 }  // namespace
@@ -572,6 +605,87 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
     ++h->order_age;
   }
   timing_end(h, h->ev_fwd, st);
+  return 0;
+}
+
+int dbmp_forward_noalloc(int image_height, int image_width, int tile_size, float alpha_upper_bound,
+                         const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                         const dbmp_tile_lists* lists, float eps_T, float* out_color,
+                         float* out_alpha, float* out_T_final, int32_t* out_last, void* stream) {
+  int rc = check_prims(prims);
+  if (rc) return rc;
+  if ((rc = check_bitmaps(bm))) return rc;
+  if (image_height <= 0 || image_width <= 0)
+    return fail(DBMP_E_ARG, "image size must be positive, got %dx%d", image_height, image_width);
+  if (tile_size <= 0 || tile_size % kSub != 0)
+    return fail(DBMP_E_ARG, "tile_size must be a positive multiple of %d, got %d", kSub, tile_size);
+  if (!(alpha_upper_bound >= 0.f))
+    return fail(DBMP_E_ARG, "alpha_upper_bound must be >= 0, got %g", (double)alpha_upper_bound);
+  if (!out_color || !out_alpha || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL output");
+  if (!(eps_T >= 0.f && eps_T < 1.f)) return fail(DBMP_E_ARG, "eps_T must be in [0, 1)");
+  const int N = prims->num_primitives;
+  if (N > kNoAllocMaxPrims)
+    return fail(DBMP_E_ARG, "dbmp_forward_noalloc: %d primitives exceed its maximum of %d", N,
+                kNoAllocMaxPrims);
+  if (bm->num_templates > kNoAllocMaxTemplates)
+    return fail(DBMP_E_ARG, "dbmp_forward_noalloc: %d templates exceed its maximum of %d",
+                bm->num_templates, kNoAllocMaxTemplates);
+  const bool rgba = wants_rgba(bm);
+  const size_t qbytes = (size_t)bm->num_templates * bm->height * bm->width *
+                        (rgba ? rgba_bytes(bm->dtype) : quad_bytes(bm->dtype));
+  if (qbytes > kNoAllocQuadBytes)
+    return fail(DBMP_E_ARG, "dbmp_forward_noalloc: texel records of %zu bytes exceed its maximum of %zu",
+                qbytes, kNoAllocQuadBytes);
+  if (!lists || !lists->tile_offsets) return fail(DBMP_E_ARG, "tile lists are NULL");
+  const int f = lists->bin_factor;
+  if (f > 1 && tile_size != kSub) return fail(DBMP_E_ARG, "bin_factor > 1 needs tile size 16");
+  if ((f > 1 || lists->pair_count) && !lists->ranges)
+    return fail(DBMP_E_ARG, "dbmp_forward_noalloc: lists with a pair count or bin_factor > 1 need "
+                "their binning's ranges");
+  if (f < 0 || (f > 1 && f != 2 && f != 4 && f != 8)) return fail(DBMP_E_ARG, "bad bin_factor %d", f);
+  unsigned char* base = nullptr;
+  if ((rc = scratch_address(&base))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  // a handle on the stack whose scratch pointers address the static array
+  // (nothing below allocates: no ensure_* call)
+  dbmp_rasterizer h;
+  h.H = image_height;
+  h.W = image_width;
+  h.ts = tile_size;
+  h.tilesX = (image_width + tile_size - 1) / tile_size;
+  h.tilesY = (image_height + tile_size - 1) / tile_size;
+  h.tiles = h.tilesX * h.tilesY;
+  h.subs = (tile_size / kSub) * (tile_size / kSub);
+  h.alpha_ub = alpha_upper_bound;
+  h.recs = (PrimRec*)base;
+  h.cull = (float4*)(base + kNoAllocRecBytes);
+  h.quads = base + kNoAllocRecBytes + kNoAllocCullBytes;
+  h.quads_used = qbytes;
+  // records, texel records and content boxes of this call (the previous
+  // call's contents are never reused)
+  dbmp_prims p = *prims;
+  p.reuse_records = 0;
+  if ((rc = launch_prep(&h, &p, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
+  if ((rc = rgba ? launch_build_rgba(bm, h.quads, st) : launch_build_quads(bm, h.quads, st))) return rc;
+  const dim3 cgrid(bm->num_templates), cblock(256);
+  if (bm->dtype == DBMP_F32)
+    hipLaunchKernelGGL((k_template_extents<float, true>), cgrid, cblock, 0, st,
+                       (const float*)bm->templates, bm->height, bm->width, (float*)h.cull);
+  else if (bm->dtype == DBMP_F16)
+    hipLaunchKernelGGL((k_template_extents<__half, true>), cgrid, cblock, 0, st,
+                       (const __half*)bm->templates, bm->height, bm->width, (float*)h.cull);
+  else
+    hipLaunchKernelGGL((k_template_extents<__hip_bfloat16, true>), cgrid, cblock, 0, st,
+                       (const __hip_bfloat16*)bm->templates, bm->height, bm->width, (float*)h.cull);
+  LAUNCH_CHECK("k_template_extents");
+  RasterArgs A = make_args(&h, &p, bm, lists, eps_T);
+  A.order = nullptr;
+  A.cost = nullptr;
+  const int nb = A.tiles * A.subs;
+  const bool blend = A.cmap != nullptr;
+  DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(nb), dim3(kThreads), 0, st, A, out_color,
+                    out_alpha, out_T_final, out_last);
+  LAUNCH_CHECK("k_raster_fwd");
   return 0;
 }
 

@@ -173,6 +173,24 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
                  const dbmp_tile_lists* lists, float eps_T, float* out_color, float* out_alpha,
                  float* out_T_final, int32_t* out_last, void* stream);
 
+/* dbmp_forward without a handle and without allocating: the same kernels and
+ * outputs, with this call's primitive records, texel quads (or RGBA records)
+ * and template content boxes in a file-scope device array of the library
+ * (about 20 MiB, one per GPU, held until the process ends).  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Limits: at most 65536 primitives, 4096 templates and 16 MiB
+ * of texel records (T*Ht*Wt*16 bytes for fp32 alpha bitmaps, *8 for fp16 /
+ * bf16, *64 / *32 with per-template colour maps); larger calls return
+ * DBMP_E_ARG.  Lists as for dbmp_forward, except that lists with a pair count
+ * or bin_factor > 1 must carry their ranges.  The array keeps the last call's
+ * contents and every call rewrites what it reads; calls that overlap on one
+ * GPU (several streams or threads) race on it. */
+int dbmp_forward_noalloc(int image_height, int image_width, int tile_size, float alpha_upper_bound,
+                         const dbmp_prims* prims, const dbmp_bitmaps* bm,
+                         const dbmp_tile_lists* lists, float eps_T, float* out_color,
+                         float* out_alpha, float* out_T_final, int32_t* out_last, void* stream);
+
 /* Backward; replaces _C.rasterize_tiles_backward_class (ext.cpp:31-50 ->
  * tile_rasterize.cu:638-780).  grad_color (H,W,3), grad_alpha (H,W) or NULL;
  * out_color / out_T_final / out_last are dbmp_forward's outputs for the same
