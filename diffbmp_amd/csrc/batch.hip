@@ -199,7 +199,7 @@ int launch_step(dbmp_batch* b, hipStream_t st) {
                        b->djobs, b->dstate, block_start(b), prim_start(b), J, b->grad8);
   LAUNCH_CHECK("k_batch_bwd");
   hipLaunchKernelGGL(k_batch_tick, dim3(ceil_div(J, 64)), dim3(64), 0, st, b->djobs, b->dstate, J,
-                     (double)b->beta1, (double)b->beta2, (const long long*)b->dtot,
+                     b->beta1, b->beta2, (const long long*)b->dtot,
                      b->host_total_dev);
   LAUNCH_CHECK("k_batch_tick");
   if (N > 0) {
@@ -416,6 +416,15 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     }
     std::memset(&hs[i], 0, sizeof(BatchState));
     hs[i].step = s.steps_done;
+    // beta^steps_done by the same fp32 products k_batch_tick forms, so a job
+    // re-admitted mid-fit continues the sequence it had
+    float b1t = 1.f, b2t = 1.f;
+    for (int t = 0; t < s.steps_done; ++t) {
+      b1t *= b->beta1;
+      b2t *= b->beta2;
+    }
+    hs[i].b1t = b1t;
+    hs[i].b2t = b2t;
   }
   // one pinned upload: jobs | state | starts (3 x (job_cap + 1) ints)
   const size_t bj = sizeof(BatchJob) * J, bst = sizeof(BatchState) * J;

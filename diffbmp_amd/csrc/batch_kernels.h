@@ -51,6 +51,7 @@ struct BatchJob {
 // Per-job state that the step itself advances.
 struct BatchState {
   int step;            // Adam steps taken
+  float b1t, b2t;      // beta1^step, beta2^step as fp32 running products
   float step_scale[5]; // lr_g / (1 - beta1^t) of the step in flight
   float inv_bc2_sqrt;  // 1 / sqrt(1 - beta2^t)
 };
@@ -170,22 +171,25 @@ static __global__ __launch_bounds__(kThreads) void k_batch_bwd(const BatchJob* _
                                     B.last, grad8 + 8L * prim_start[j]);
 }
 
-// t <- t + 1 and the bias corrections of step t, as torch.optim.Adam forms
-// them in double precision.
+// t <- t + 1 and the bias corrections 1 - beta^t of step t
+// (torch.optim.Adam), from fp32 running products of beta (torch forms
+// beta^t in double on the host; the products agree to a few fp32 ulp over
+// the step counts of a fit).
 // Thread 0 also copies the step's pair count to host-mapped memory.
 static __global__ void k_batch_tick(const BatchJob* __restrict__ jobs, BatchState* __restrict__ state,
-                             int J, double beta1, double beta2, const long long* __restrict__ total,
+                             int J, float beta1, float beta2, const long long* __restrict__ total,
                              long long* __restrict__ host_total) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0 && host_total) *host_total = *total;
   if (j >= J) return;
-  const int t = state[j].step + 1;
-  state[j].step = t;
-  const double bc1 = 1.0 - pow(beta1, (double)t);
-  const double bc2 = 1.0 - pow(beta2, (double)t);
+  BatchState& S = state[j];
+  S.step += 1;
+  S.b1t *= beta1;
+  S.b2t *= beta2;
+  const float bc1 = 1.f - S.b1t, bc2 = 1.f - S.b2t;
 #pragma unroll
-  for (int i = 0; i < 5; ++i) state[j].step_scale[i] = (float)((double)jobs[j].lr[i] / bc1);
-  state[j].inv_bc2_sqrt = (float)(1.0 / sqrt(bc2));
+  for (int i = 0; i < 5; ++i) S.step_scale[i] = jobs[j].lr[i] / bc1;
+  S.inv_bc2_sqrt = 1.f / sqrtf(bc2);
 }
 
 // One thread per parameter value; value q of a row belongs to group

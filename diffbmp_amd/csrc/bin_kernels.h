@@ -20,6 +20,7 @@
 //                 compaction), so the lists equal the sorted ones.
 #pragma once
 #include "dbmp_device.h"
+#include "dbmp_trig.h"
 
 namespace dbmp {
 
@@ -120,8 +121,9 @@ __device__ __forceinline__ void prep_core(float x, float y, float r, float th, f
                                           int T, int Ht, int Wt, int H, int W, int ts, int tilesX,
                                           int tilesY, float alpha_ub, int bin_shift, PrimRec& p,
                                           int& cnt, int4& rg) {
-  // correctly rounded fp32 trig, shared by binning and raster
-  const float c = (float)cos((double)th), s = (float)sin((double)th);
+  // fp32 trig shared by binning, raster and the oracle (dbmp_trig.h)
+  float c, s;
+  dbmp_sincosf(th, &s, &c);
   p.mx = x; p.my = y; p.r = r; p.c = c; p.s = s;
   const float tw = (float)Wt, tht = (float)Ht;
   const float md = fmaxf(tw, tht);

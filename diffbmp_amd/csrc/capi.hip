@@ -76,6 +76,12 @@ int scratch_address(unsigned char** out) {
   return 0;
 }
 
+static __global__ void k_sincos(const float* __restrict__ x, float* __restrict__ s,
+                                float* __restrict__ c, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dbmp_sincosf(x[i], s + i, c + i);
+}
+
 }  // namespace
 
 struct dbmp_rasterizer {
@@ -853,6 +859,16 @@ int dbmp_adam_step(dbmp_rasterizer* h, const dbmp_adam* cfg, void* stream) {
   const int gx = (int)std::min<long long>(ceil_div(nmax, 256), 1024);
   hipLaunchKernelGGL(k_adam, dim3(gx, cfg->num_groups), dim3(256), 0, (hipStream_t)stream, A);
   LAUNCH_CHECK("k_adam");
+  return 0;
+}
+
+int dbmp_sincos(const float* x, float* s, float* c, int64_t n, void* stream) {
+  if (n < 0) return fail(DBMP_E_ARG, "n < 0");
+  if (n == 0) return 0;
+  if (!x || !s || !c) return fail(DBMP_E_ARG, "NULL pointer");
+  hipLaunchKernelGGL(k_sincos, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, x, s, c,
+                     (long long)n);
+  LAUNCH_CHECK("k_sincos");
   return 0;
 }
 

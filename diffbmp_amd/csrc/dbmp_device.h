@@ -20,7 +20,7 @@ enum DType : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
 // the fourth holds what only the binning and the gradient reduction use.
 struct alignas(16) PrimRec {
   float mx, my, hw, hh;      // centre; push-box half extents (tile_forward.cu:195-210)
-  float c, s, rinv, abase;   // cos, sin (fp32, correctly rounded), 1/r, alpha_ub * sigmoid(v)
+  float c, s, rinv, abase;   // cos, sin (dbmp_sincosf), 1/r, alpha_ub * sigmoid(v)
   float cr, cg, cb;          // sigmoid(colour logits)
   int tbase;                 // tmpl * Ht * Wt: first texel quad of the template
   int tmpl;                  // template index, -1 => never pushed

@@ -211,7 +211,7 @@ constexpr int kPsdPix = 4 * kPsdThreads;   // pixels per workgroup
 
 // Packed crops: thread t writes pixels 4t..4t+3 as one 16-byte store.  The
 // workgroup finds the layers its 1024 pixels span once; cos/sin of their
-// angles (correctly rounded fp32, through double) go to LDS, one layer per
+// angles (dbmp_sincosf, fp32) go to LDS, one layer per
 // thread, since a workgroup spans at most one layer per pixel.  Optionally
 // records each layer's largest alpha byte (the host's "has some alpha" test,
 // psd_export_uint8.py:140-143).
@@ -229,9 +229,7 @@ static __global__ __launch_bounds__(kPsdThreads) void k_psd_layers(PsdArgs P, co
   __syncthreads();
   const int nlo = span[0], nhi = span[1];
   for (int k = threadIdx.x; k <= nhi - nlo; k += kPsdThreads) {
-    const double th = (double)P.rotations[nlo + k];
-    scos[k] = (float)cos(th);
-    ssin[k] = (float)sin(th);
+    dbmp_sincosf(P.rotations[nlo + k], &ssin[k], &scos[k]);
   }
   __syncthreads();
   const long long g = g0 + 4LL * threadIdx.x;

@@ -374,13 +374,21 @@ int dbmp_psd_bounds(const dbmp_layer_prims* prims, int canvas_height, int canvas
  * the reference's gathered layout), 2 per template (T maps, indexed by
  * bmp_sel); blended as (1 - c_blend) c + c_blend c_o when c_blend > 0.
  * layer_alpha_max (N) int32 or NULL: each layer's largest alpha byte.
- * Arithmetic is the reference's, in IEEE fp32 with correctly rounded
- * cos/sin and division (the reference builds with fast math). */
+ * Arithmetic is the reference's, in IEEE fp32 with IEEE division and the
+ * library's fp32 cos/sin (dbmp_sincos; the reference builds with fast math). */
 int dbmp_psd_layers(const dbmp_layer_prims* prims, const float* templates, int num_templates,
                     int tmpl_h, int tmpl_w, const float* color_maps, int color_map_mode, float c_blend,
                     int canvas_height, int canvas_width, float scale_factor, float alpha_upper_bound,
                     const int32_t* boxes, const int64_t* byte_offsets, int64_t total_bytes,
                     uint8_t* out, int32_t* layer_alpha_max, void* stream);
+
+/* The fp32 sine and cosine every kernel of this library uses (binning,
+ * raster, layer export): s[i] = sin(x[i]), c[i] = cos(x[i]) for n angles,
+ * at most 2 ulp from the exact values, no fp64.  No counterpart in the
+ * reference (it uses torch.cos for binning and __cosf in its kernels); it is
+ * exposed so that a host-side restatement of the binning can use the same
+ * values (the oracle compiles the same source, csrc/dbmp_trig.h). */
+int dbmp_sincos(const float* x, float* s, float* c, int64_t n, void* stream);
 
 #ifdef __cplusplus
 }

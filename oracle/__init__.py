@@ -90,10 +90,15 @@ def template_extents_np(templates: np.ndarray) -> np.ndarray:
     return out
 
 
-def cos_sin_rn(theta: np.ndarray):
-    """Correctly rounded fp32 cos/sin (fp64 evaluation, one rounding)."""
-    t = theta.astype(np.float64)
-    return np.cos(t).astype(f32), np.sin(t).astype(f32)
+def cos_sin(theta: np.ndarray):
+    """fp32 cos/sin of ``theta`` by ``dbmp_sincosf`` (``diffbmp_amd/csrc/dbmp_trig.h``,
+    compiled into the C oracle): the same values the GPU kernels use, so the
+    binning decisions below are the kernels'."""
+    t = np.ascontiguousarray(theta, f32).ravel()
+    s = np.empty_like(t)
+    c = np.empty_like(t)
+    lib("f64").oracle_sincos_array(len(t), _p(t), _p(s), _p(c))
+    return c.reshape(np.shape(theta)), s.reshape(np.shape(theta))
 
 
 def bin_bboxes_np(means2D, radii, rotations, sel, ext) -> np.ndarray:
@@ -102,7 +107,7 @@ def bin_bboxes_np(means2D, radii, rotations, sel, ext) -> np.ndarray:
     x = means2D[:, 0].astype(f32)[:, None]
     y = means2D[:, 1].astype(f32)[:, None]
     r = radii.astype(f32)[:, None]
-    c, s = cos_sin_rn(rotations.astype(f32))
+    c, s = cos_sin(rotations.astype(f32))
     c, s = c[:, None], s[:, None]
     e = ext[np.clip(sel, 0, ext.shape[0] - 1)]
     cu = np.stack([e[:, 0], e[:, 1], e[:, 0], e[:, 1]], 1).astype(f32)

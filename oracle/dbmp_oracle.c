@@ -28,8 +28,9 @@
  *                helper tile_common.h:201-239
  *
  * Deliberate, documented differences from the reference's CUDA build:
- *   - cos/sin are the correctly rounded fp32 values (computed in fp64 and
- *     rounded); the reference uses torch.cos (binning) and __cosf (kernel).
+ *   - cos/sin are dbmp_sincosf (diffbmp_amd/csrc/dbmp_trig.h, fp32, <= 2 ulp),
+ *     the same source the GPU kernels compile, so both make the same fp32
+ *     decisions; the reference uses torch.cos (binning) and __cosf (kernel).
  *   - 1/r is IEEE division; the reference builds with --use_fast_math.
  *   - no FMA contraction in fp32 decision arithmetic (-ffp-contract=off).
  *   - the per-pixel cache has no capacity limit (reference defect: overflow
@@ -42,6 +43,8 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+#include "../diffbmp_amd/csrc/dbmp_trig.h"
 
 #ifndef REAL
 #define REAL double
@@ -86,8 +89,43 @@ static REAL sigm(REAL x) {
     return z / ((REAL)1 + z);
 }
 
-static float cos_rn(float t) { return (float)cos((double)t); }
-static float sin_rn(float t) { return (float)sin((double)t); }
+/* fp32 cos / sin shared with the GPU kernels (dbmp_trig.h) */
+static float cos_f(float t) { float s, c; dbmp_sincosf(t, &s, &c); return c; }
+static float sin_f(float t) { float s, c; dbmp_sincosf(t, &s, &c); return s; }
+
+int oracle_sincos_array(int n, const float *x, float *s, float *c) {
+    for (int i = 0; i < n; ++i) dbmp_sincosf(x[i], s + i, c + i);
+    return 0;
+}
+
+/* Largest error in ulp of the fp32 result of dbmp_sincosf against the C
+ * library's double sin / cos, over the fp32 values with bit patterns
+ * lo, lo + stride, ... < hi (positive angles).  out[0] sin, out[1] cos. */
+static double ulp_err(float got, double want) {
+    const float w = (float)want;
+    int e;
+    frexpf(fabsf(w) > 0.f ? w : 1e-38f, &e);
+    double ulp = ldexp(1.0, e - 24);
+    if (ulp < ldexp(1.0, -149)) ulp = ldexp(1.0, -149);
+    return fabs((double)got - want) / ulp;
+}
+
+int oracle_sincos_scan(unsigned lo, unsigned hi, unsigned stride, double *out) {
+    double ms = 0, mc = 0;
+#pragma omp parallel for reduction(max : ms, mc) schedule(static)
+    for (long long b = lo; b < (long long)hi; b += stride) {
+        union { unsigned u; float f; } v;
+        v.u = (unsigned)b;
+        float s, c;
+        dbmp_sincosf(v.f, &s, &c);
+        const double es = ulp_err(s, sin((double)v.f)), ec = ulp_err(c, cos((double)v.f));
+        if (es > ms) ms = es;
+        if (ec > mc) mc = ec;
+    }
+    out[0] = ms;
+    out[1] = mc;
+    return 0;
+}
 
 /* One sample of primitive n at integer pixel (px,py).  Returns 1 if the
  * sample is pushed (passes the template box test and has mask > 1e-6),
@@ -176,7 +214,7 @@ static int sample_prim(const dbmp_oracle_scene *s, int n, int px, int py, sample
         if (fabsf(dx) > hw || fabsf(dy) > hh) return 0;
         const float rinv = r > 1e-6f ? 1.f / r : 1e6f;
         const float phi = s->rotations[n];
-        const float c = cos_rn(phi), sn = sin_rn(phi);
+        const float c = cos_f(phi), sn = sin_f(phi);
         const float ndx = dx * rinv, ndy = dy * rinv;
         const float u = c * ndx + sn * ndy;
         const float v = -sn * ndx + c * ndy;
@@ -263,7 +301,7 @@ int oracle_template_extents(const float *tmpl, int T, int Ht, int Wt, float *ext
 /* ------------------------------------------------------------------ */
 static void world_bbox(float x, float y, float r, float th, const float *e,
                        float *xmin, float *xmax, float *ymin, float *ymax) {
-    const float c = cos_rn(th), s = sin_rn(th);
+    const float c = cos_f(th), s = sin_f(th);
     const float cu[4] = {e[0], e[1], e[0], e[1]};
     const float cv[4] = {e[2], e[2], e[3], e[3]};
     float wx[4], wy[4];

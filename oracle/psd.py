@@ -12,8 +12,9 @@ between them (paths relative to the reference tree):
   ``psd_grid_sample_bilinear`` (``psd_export_common.h:58-108``),
   ``psd_sample_color_map`` (``:129-196``) and ``float_to_uint8`` (``:123-125``).
 
-cos/sin are correctly rounded to fp32 (fp64 evaluation, one rounding) and the
-divisions are IEEE; the reference builds with fast math, whose approximate
+cos/sin are the library's fp32 ``dbmp_sincosf`` (the source the GPU kernels
+compile, through the C oracle) and the divisions are IEEE; the reference
+builds with fast math, whose approximate
 trig and division this restatement does not reproduce.  How each function is
 pinned: ``tests/test_psd_oracle.py`` (hand-computed boxes, closed forms for
 constant and texel-aligned templates, a rotation that must mirror an
@@ -23,7 +24,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import cos_sin_rn
+from . import cos_sin
 
 f32 = np.float32
 
@@ -134,7 +135,7 @@ def psd_layers(means2D, radii, rotations, colors, visibility, bmp_sel, templates
             continue   # no template: transparent
         py, px = np.mgrid[y0:y1, x0:x1]
         x, y, r = m[n, 0] * sc, m[n, 1] * sc, rad[n] * sc
-        c, s = cos_sin_rn(rot[n:n + 1])
+        c, s = cos_sin(rot[n:n + 1])
         c, s = c[0], s[0]
         tv = _template_value(tms[t], Ht, Wt, px, py, x, y, r, c, s)
         alpha = tv * vis[n] * f32(alpha_ub)
