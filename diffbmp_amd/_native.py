@@ -135,6 +135,16 @@ EXPORTS = {
                                             ctypes.POINTER(TileLists), ctypes.c_float,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_forward_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Prims),
+                                          ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
+                                          ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dbmp_backward_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Prims),
+                                           ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
+                                           ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dbmp_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims),
                                      ctypes.POINTER(Bitmaps), ctypes.POINTER(TileLists),
                                      ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
@@ -437,6 +447,31 @@ class Handle:
         check(self._L.dbmp_forward(self._h, ctypes.byref(prims), ctypes.byref(bm), ctypes.byref(L),
                                    float(eps_T), _ptr(out_color), _ptr(out_alpha), _ptr(out_T),
                                    _ptr(out_last), _stream(out_color.device)))
+
+    def forward_batch(self, prims: list, bm: Bitmaps, lists: list, eps_T: float, out_color,
+                      out_alpha, out_T, out_last) -> None:
+        """B candidates (same N, dtype, bitmaps) in one launch; outputs stacked
+        (B,H,W,3) / (B,H,W)."""
+        B = len(prims)
+        P = (Prims * B)(*prims)
+        L = (TileLists * B)(*[x.c() for x in lists])
+        self.records_gen += 1
+        check(self._L.dbmp_forward_batch(self._h, B, P, ctypes.byref(bm), L, float(eps_T),
+                                         _ptr(out_color), _ptr(out_alpha), _ptr(out_T),
+                                         _ptr(out_last), _stream(out_color.device)))
+
+    def backward_batch(self, prims: list, bm: Bitmaps, lists: list, eps_T, grad_color, grad_alpha,
+                       out_color, out_T, out_last, g_means, g_r, g_th, g_v, g_c) -> None:
+        """Backward of forward_batch; gradients stacked (B,N,...)."""
+        B = len(prims)
+        P = (Prims * B)(*prims)
+        L = (TileLists * B)(*[x.c() for x in lists])
+        self.records_gen += 1
+        check(self._L.dbmp_backward_batch(self._h, B, P, ctypes.byref(bm), L, float(eps_T),
+                                          _ptr(grad_color), _ptr(grad_alpha), _ptr(out_color),
+                                          _ptr(out_T), _ptr(out_last), _ptr(g_means), _ptr(g_r),
+                                          _ptr(g_th), _ptr(g_v), _ptr(g_c),
+                                          _stream(grad_color.device)))
 
     def backward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T, grad_color, grad_alpha,
                  out_color, out_T, out_last, g_means, g_r, g_th, g_v, g_c) -> None:

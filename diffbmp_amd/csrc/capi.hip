@@ -212,17 +212,19 @@ int check_bitmaps(const dbmp_bitmaps* b) {
 
 // ranges: where the binning's tile ranges go (h->ranges when null); shift:
 // log2 of the super-tile factor the pair counts are for.
+// recs_at: write the records there instead of h->recs (candidate b of a
+// batch at h->recs + b N); such records are never reused by later calls.
 int launch_prep(dbmp_rasterizer* h, const dbmp_prims* p, const float* ext, int T, int Ht, int Wt,
-                hipStream_t st, int4* ranges = nullptr, int shift = 0) {
+                hipStream_t st, int4* ranges = nullptr, int shift = 0, PrimRec* recs_at = nullptr) {
   const int N = p->num_primitives;
-  if (ext == nullptr && p->reuse_records && h->recs_n == N) return 0;
-  h->recs_n = N;
+  if (recs_at == nullptr && ext == nullptr && p->reuse_records && h->recs_n == N) return 0;
+  h->recs_n = recs_at == nullptr ? N : -1;
   if (N == 0) return 0;
   const dim3 grid(ceil_div(N, 256)), block(256);
 #define PREP_ARGS(TY)                                                                          \
   N, (const TY*)p->means2D, (const TY*)p->radii, (const TY*)p->rotations,                      \
       (const TY*)p->opacities, (const TY*)p->colors, p->bmp_sel, ext, T, Ht, Wt, h->H, h->W,   \
-      h->ts, h->tilesX, h->tilesY, h->alpha_ub, shift, h->recs, h->counts,                    \
+      h->ts, h->tilesX, h->tilesY, h->alpha_ub, shift, recs_at ? recs_at : h->recs, h->counts, \
       ranges ? ranges : h->ranges
   if (p->dtype == DBMP_F32)
     hipLaunchKernelGGL(k_prep<float>, grid, block, 0, st, PREP_ARGS(float));
@@ -692,6 +694,146 @@ int dbmp_forward_noalloc(int image_height, int image_width, int tile_size, float
   DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd, dim3(nb), dim3(kThreads), 0, st, A, out_color,
                     out_alpha, out_T_final, out_last);
   LAUNCH_CHECK("k_raster_fwd");
+  return 0;
+}
+
+// Shared checks of the candidate-batch calls: B >= 1 candidates of one N and
+// dtype, lists that carry their ranges wherever an overflow walk could need them.
+static int check_candidates(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
+                            const dbmp_tile_lists* lists) {
+  if (!h) return fail(DBMP_E_ARG, "handle is NULL");
+  if (B < 1 || !prims || !lists) return fail(DBMP_E_ARG, "need B >= 1 candidates with prims and lists");
+  int rc;
+  for (int b = 0; b < B; ++b) {
+    if ((rc = check_prims(&prims[b]))) return rc;
+    if ((rc = check_lists(h, &lists[b]))) return rc;
+    if (prims[b].num_primitives != prims[0].num_primitives || prims[b].dtype != prims[0].dtype)
+      return fail(DBMP_E_ARG, "candidate %d: every candidate needs the same N and dtype", b);
+    if (lists[b].bin_factor != lists[0].bin_factor)
+      return fail(DBMP_E_ARG, "candidate %d: every candidate needs the same bin_factor", b);
+    if (lists[b].pair_count && !lists[b].ranges)
+      return fail(DBMP_E_ARG, "candidate %d: lists with a pair count need their ranges", b);
+  }
+  if ((long long)B * prims[0].num_primitives >= (1LL << 30))
+    return fail(DBMP_E_ARG, "B * N too large");
+  return 0;
+}
+
+// Per-candidate table of a launch (at most kMaxCands candidates).
+static CandTable cand_table(const dbmp_tile_lists* lists, const dbmp_bitmaps* bm, int b0, int Bc,
+                            long long cmap_stride) {
+  CandTable C;
+  for (int i = 0; i < kMaxCands; ++i) {
+    const int b = b0 + (i < Bc ? i : 0);
+    const dbmp_tile_lists& L = lists[b];
+    C.toffs[i] = L.tile_offsets;
+    C.tidx[i] = L.tile_indices;
+    C.pcount[i] = (const long long*)L.pair_count;
+    C.pcap[i] = L.capacity;
+    C.ranges[i] = (const int4*)L.ranges;
+    C.cmap[i] = bm->colors_orig_mode == 1 && bm->colors_orig
+                    ? (const void*)((const char*)bm->colors_orig + cmap_stride * b)
+                    : nullptr;
+  }
+  return C;
+}
+
+// Bytes between candidates' per-primitive colour maps (mode 1): each
+// candidate's (N,Ht,Wt,3) block in the bitmaps' dtype.
+static long long cmap_cand_stride(const dbmp_bitmaps* bm, int N) {
+  return (long long)N * bm->height * bm->width * 3 * (bm->dtype == DBMP_F32 ? 4 : 2);
+}
+
+int dbmp_forward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
+                       const dbmp_bitmaps* bm, const dbmp_tile_lists* lists, float eps_T,
+                       float* out_color, float* out_alpha, float* out_T_final, int32_t* out_last,
+                       void* stream) {
+  int rc = check_candidates(h, B, prims, lists);
+  if (rc) return rc;
+  if ((rc = check_bitmaps(bm))) return rc;
+  if (!out_color || !out_alpha || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL output");
+  if (!(eps_T >= 0.f && eps_T < 1.f)) return fail(DBMP_E_ARG, "eps_T must be in [0, 1)");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = prims[0].num_primitives;
+  if ((rc = ensure_n(h, std::max(B * N, 1)))) return rc;
+  timing_begin(h, h->ev_fwd, st);
+  if ((rc = ensure_quads(h, bm, st))) return rc;
+  for (int b = 0; b < B; ++b)
+    if ((rc = launch_prep(h, &prims[b], nullptr, bm->num_templates, bm->height, bm->width, st,
+                          nullptr, 0, h->recs + (long long)b * N)))
+      return rc;
+  RasterArgs A = make_args(h, &prims[0], bm, &lists[0], eps_T);
+  A.order = nullptr;
+  A.cost = nullptr;
+  const int nb = A.tiles * A.subs;
+  const long long px = (long long)h->H * h->W;
+  const bool blend = A.cmap != nullptr;
+  const long long cst = cmap_cand_stride(bm, N);
+  for (int b0 = 0; b0 < B; b0 += kMaxCands) {
+    const int Bc = std::min(kMaxCands, B - b0);
+    RasterArgs Ab = A;
+    Ab.recs = h->recs + (long long)b0 * N;
+    const CandTable C = cand_table(lists, bm, b0, Bc, cst);
+    DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_fwd_cands, dim3(nb * Bc), dim3(kThreads), 0, st, Ab,
+                      C, nb, out_color + 3 * px * b0, out_alpha + px * b0, out_T_final + px * b0,
+                      out_last + px * b0);
+    LAUNCH_CHECK("k_raster_fwd_cands");
+  }
+  timing_end(h, h->ev_fwd, st);
+  return 0;
+}
+
+int dbmp_backward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
+                        const dbmp_bitmaps* bm, const dbmp_tile_lists* lists, float eps_T,
+                        const float* grad_color, const float* grad_alpha, const float* out_color,
+                        const float* out_T_final, const int32_t* out_last, float* grad_means2D,
+                        float* grad_radii, float* grad_rotations, float* grad_opacities,
+                        float* grad_colors, void* stream) {
+  (void)eps_T;
+  int rc = check_candidates(h, B, prims, lists);
+  if (rc) return rc;
+  if ((rc = check_bitmaps(bm))) return rc;
+  if (!grad_color || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
+  if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
+    return fail(DBMP_E_ARG, "NULL gradient output");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = prims[0].num_primitives;
+  const long long BN = (long long)B * N;
+  if ((rc = ensure_n(h, (int)std::max<long long>(BN, 1)))) return rc;
+  timing_begin(h, h->ev_bwd, st);
+  if ((rc = ensure_quads(h, bm, st))) return rc;
+  for (int b = 0; b < B; ++b)
+    if ((rc = launch_prep(h, &prims[b], nullptr, bm->num_templates, bm->height, bm->width, st,
+                          nullptr, 0, h->recs + (long long)b * N)))
+      return rc;
+  if (BN > 0) {
+    hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8 * BN, 256)), dim3(256), 0, st, h->grad8, 8 * BN, 0.f);
+    LAUNCH_CHECK("k_fill_f32");
+  }
+  RasterArgs A = make_args(h, &prims[0], bm, &lists[0], 0.f);
+  A.order = nullptr;
+  A.cost = nullptr;
+  const int nb = A.tiles * A.subs;
+  const long long px = (long long)h->H * h->W;
+  const bool blend = A.cmap != nullptr;
+  const long long cst = cmap_cand_stride(bm, N);
+  for (int b0 = 0; b0 < B; b0 += kMaxCands) {
+    const int Bc = std::min(kMaxCands, B - b0);
+    RasterArgs Ab = A;
+    Ab.recs = h->recs + (long long)b0 * N;
+    const CandTable C = cand_table(lists, bm, b0, Bc, cst);
+    DISPATCH_DT_BLEND(bm->dtype, blend, k_raster_bwd_cands, dim3(nb * Bc), dim3(kThreads), 0, st, Ab,
+                      C, nb, grad_color + 3 * px * b0, grad_alpha ? grad_alpha + px * b0 : nullptr,
+                      out_color + 3 * px * b0, out_T_final + px * b0, out_last + px * b0,
+                      h->grad8 + 8LL * N * b0);
+    LAUNCH_CHECK("k_raster_bwd_cands");
+  }
+  if (BN > 0) {
+    hipLaunchKernelGGL(k_unpack_grads, dim3(ceil_div(BN, 256)), dim3(256), 0, st, h->grad8, (int)BN,
+                       grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);
+    LAUNCH_CHECK("k_unpack_grads");
+  }
+  timing_end(h, h->ev_bwd, st);
   return 0;
 }
 

@@ -735,6 +735,58 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd(RasterArgs A, const flo
                                      none, Cfin, Tfin, lastin, grad8);
 }
 
+// Candidates of one canvas and template set rasterised by one launch
+// (the reference's batch path, tile_rasterize.cu:500-636): workgroup ->
+// (candidate, block), with each candidate's records at recs + b N and its own
+// lists, pair count, ranges and per-primitive colour maps from this table.
+constexpr int kMaxCands = 32;
+struct CandTable {
+  const int* toffs[kMaxCands];
+  const int* tidx[kMaxCands];
+  const long long* pcount[kMaxCands];
+  long long pcap[kMaxCands];
+  const int4* ranges[kMaxCands];
+  const void* cmap[kMaxCands];   // cmap_mode 1 only
+};
+
+__device__ __forceinline__ RasterArgs candidate_args(const RasterArgs& A, const CandTable& C, int b) {
+  RasterArgs Ab = A;
+  Ab.recs = A.recs + (long)b * A.N;
+  Ab.toffs = C.toffs[b];
+  Ab.tidx = C.tidx[b];
+  Ab.pcount = C.pcount[b];
+  Ab.pcap = C.pcap[b];
+  Ab.ranges = C.ranges[b];
+  if (A.cmap != nullptr && A.cmap_mode == 1) Ab.cmap = C.cmap[b];
+  return Ab;
+}
+
+template <int DT, bool BLEND>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(BLEND ? 1 : 8))) void k_raster_fwd_cands(
+    RasterArgs A, CandTable C, int nb, float* __restrict__ out_color, float* __restrict__ out_alpha,
+    float* __restrict__ out_T, int* __restrict__ out_last) {
+  // contiguous bands of (candidate, block) per XCD
+  const int r = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = r / nb, blk = r - b * nb;
+  const long px = (long)A.H * A.W;
+  raster_fwd_block<DT, BLEND>(candidate_args(A, C, b), blk, out_color + 3 * px * b,
+                              out_alpha + px * b, out_T + px * b, out_last + px * b);
+}
+
+template <int DT, bool BLEND>
+__global__ __launch_bounds__(kThreads) void k_raster_bwd_cands(
+    RasterArgs A, CandTable C, int nb, const float* __restrict__ gCin, const float* __restrict__ gAin,
+    const float* __restrict__ Cfin, const float* __restrict__ Tfin, const int* __restrict__ lastin,
+    float* __restrict__ grad8) {
+  const int r = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = r / nb, blk = r - b * nb;
+  const long px = (long)A.H * A.W;
+  const LossArgs none{};
+  raster_bwd_block<DT, BLEND, false>(candidate_args(A, C, b), blk, gCin + 3 * px * b,
+                                     gAin ? gAin + px * b : nullptr, none, Cfin + 3 * px * b,
+                                     Tfin + px * b, lastin + px * b, grad8 + 8L * A.N * b);
+}
+
 // Heaviest-first order of nb blocks from their forward costs: 64 cost
 // buckets (bucket 0 the heaviest), blocks within a bucket in any order.
 // One 1024-thread workgroup (16 waves: the kernel is latency bound, so

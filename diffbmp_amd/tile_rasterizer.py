@@ -254,9 +254,14 @@ class TileRasterizerFunctionFP16(Function):
 
 
 class TileRasterizerBatchFunction(Function):
-    """B candidates with shared templates (``cuda_tile_rasterizer/__init__.py:124-252``).
-    Each candidate keeps its own tile lists and forward state, so every
-    candidate's gradient is its own."""
+    """B candidates with shared templates (``cuda_tile_rasterizer/__init__.py:124-252``):
+    ``means2D`` (B,N,2), ``radii`` / ``rotations`` / ``opacities`` (B,N),
+    ``colors`` (B,N,3), ``colors_orig`` (B,N,Ht,Wt,3) or one (N,Ht,Wt,3) for
+    all.  Each candidate is binned on its own; then one raster launch renders
+    every candidate and one launch back-propagates all of them
+    (``dbmp_forward_batch`` / ``dbmp_backward_batch``).  Each candidate keeps
+    its own tile lists and forward state, so every candidate's gradient is
+    its own."""
 
     @staticmethod
     def forward(ctx, means2D, radii, rotations, opacities, colors, colors_orig, primitive_templates,
@@ -264,41 +269,64 @@ class TileRasterizerBatchFunction(Function):
                 tile_size, sigma, use_fp16=False):
         kind = "fp16" if use_fp16 else "fp32"
         handle = _active(kind)
+        if (int(image_height), int(image_width), int(tile_size)) != (handle.H, handle.W, handle.ts):
+            raise ValueError(f"rasterizer was initialised for {handle.H}x{handle.W} tile {handle.ts}, "
+                             f"called with {image_height}x{image_width} tile {tile_size}")
         store = getattr(handle, "store_dtype", torch.float16) if use_fp16 else torch.float32
         B = means2D.shape[0]
+        if means2D.dim() != 3 or any(t.shape[0] != B for t in (radii, rotations, opacities, colors)):
+            raise ValueError("batch parameters must be means2D (B,N,2), radii/rotations/opacities "
+                             "(B,N), colors (B,N,3)")
+        if isinstance(c_blend, torch.Tensor):
+            c_blend = float(c_blend.item())
+        c_blend = float(c_blend)
         maps = list(tile_primitive_mappings) if tile_primitive_mappings else []
-        runs, outs_c, outs_a, saved = [], [], [], []
-        for b in range(B):
-            mp = maps[b] if b < len(maps) else None
-            co = None
-            if colors_orig is not None and colors_orig.numel() > 0:
-                co = colors_orig[b] if colors_orig.dim() == 5 else colors_orig
-            run = _Run(handle, store, means2D[b], radii[b], rotations[b], opacities[b], colors[b], co,
-                       primitive_templates, global_bmp_sel, c_blend, mp,
-                       getattr(handle, "eps_T", DEFAULT_EPS_T))
-            oc, oa, oT, ol = run.forward()
-            runs.append(run)
-            outs_c.append(oc)
-            outs_a.append(oa)
-            saved += [oc, oT, ol]
-        ctx.runs = runs
-        ctx.save_for_backward(*saved)
-        out_c, out_a = torch.stack(outs_c), torch.stack(outs_a)
+        eps_T = getattr(handle, "eps_T", DEFAULT_EPS_T)
+        runs = [_Run(handle, store, means2D[b], radii[b], rotations[b], opacities[b], colors[b], None,
+                     primitive_templates, global_bmp_sel, 0.0, maps[b] if b < len(maps) else None, eps_T)
+                for b in range(B)]
+        co_all = None
+        tm = runs[0].templates
+        if colors_orig is not None and colors_orig.numel() > 0 and c_blend > 0.0:
+            N = radii.shape[1]
+            want = (N, *tm.shape[1:], 3)
+            if colors_orig.dim() == 4 and tuple(colors_orig.shape) == want:
+                colors_orig = colors_orig.unsqueeze(0).expand(B, *want)
+            if colors_orig.dim() != 5 or tuple(colors_orig.shape) != (B, *want):
+                raise ValueError(f"colors_orig must be (B,N,Ht,Wt,3) = {(B, *want)}, "
+                                 f"got {tuple(colors_orig.shape)}")
+            co_all = colors_orig.to(dtype=store).contiguous().view(B * N, *want[1:])
+        bm = Handle.bitmaps(tm, co_all, 1 if co_all is not None else 0, c_blend, runs[0].tmpl_version,
+                            bool(getattr(handle, "blend_grad_exact", False)))
+        dev, H, W = runs[0].means2D.device, handle.H, handle.W
+        out_c = torch.empty((B, H, W, 3), dtype=torch.float32, device=dev)
+        out_a = torch.empty((B, H, W), dtype=torch.float32, device=dev)
+        out_T = torch.empty((B, H, W), dtype=torch.float32, device=dev)
+        out_l = torch.empty((B, H, W), dtype=torch.int32, device=dev)
+        handle.forward_batch([r.prims() for r in runs], bm, [r.lists for r in runs], eps_T,
+                             out_c, out_a, out_T, out_l)
+        ctx.runs, ctx.bm, ctx.co_all, ctx.eps_T = runs, bm, co_all, eps_T
+        ctx.save_for_backward(out_c, out_T, out_l)
+        ctx.mark_non_differentiable(out_T, out_l)
         if use_fp16:
             return out_c.to(store), out_a.to(store)
         return out_c, out_a
 
     @staticmethod
     def backward(ctx, grad_out_color, grad_out_alpha):
-        saved = ctx.saved_tensors
-        gs = [[] for _ in range(5)]
-        for b, run in enumerate(ctx.runs):
-            oc, oT, ol = saved[3 * b: 3 * b + 3]
-            gC = grad_out_color[b] if grad_out_color is not None else torch.zeros_like(oc)
-            gA = grad_out_alpha[b] if grad_out_alpha is not None else None
-            for i, g in enumerate(run.backward(gC, gA, oc, oT, ol)):
-                gs[i].append(g)
-        return tuple(torch.stack(g) for g in gs) + (None,) * 11
+        out_c, out_T, out_l = ctx.saved_tensors
+        runs = ctx.runs
+        B, N = len(runs), runs[0].radii.shape[0]
+        dev, f32 = out_c.device, torch.float32
+        gC = (torch.zeros_like(out_c) if grad_out_color is None
+              else grad_out_color.to(f32).contiguous())
+        gA = None if grad_out_alpha is None else grad_out_alpha.to(f32).contiguous()
+        g = [torch.empty((B, N, 2), dtype=f32, device=dev), torch.empty((B, N), dtype=f32, device=dev),
+             torch.empty((B, N), dtype=f32, device=dev), torch.empty((B, N), dtype=f32, device=dev),
+             torch.empty((B, N, 3), dtype=f32, device=dev)]
+        runs[0].handle.backward_batch([r.prims() for r in runs], ctx.bm, [r.lists for r in runs],
+                                      ctx.eps_T, gC, gA, out_c, out_T, out_l, *g)
+        return tuple(g) + (None,) * 11
 
 
 class TileRasterizer:

@@ -191,6 +191,32 @@ int dbmp_forward_noalloc(int image_height, int image_width, int tile_size, float
                          const dbmp_tile_lists* lists, float eps_T, float* out_color,
                          float* out_alpha, float* out_T_final, int32_t* out_last, void* stream);
 
+/* B candidates of one canvas and template set in one raster launch per
+ * stage; replaces _C.rasterize_tiles_batch_class and
+ * rasterize_tiles_backward_batch_class (ext.cpp:53-107 ->
+ * tile_rasterize.cu:500-636).  prims[b] and lists[b] are candidate b's
+ * parameters (the same N and dtype for every candidate) and tile lists (the
+ * same bin_factor; lists with a pair count must carry their ranges); with
+ * per-primitive colour maps (colors_orig_mode 1) bm->colors_orig holds the
+ * candidates' (N,Ht,Wt,3) maps one after another.  Outputs and per-pixel
+ * state are stacked: out_color (B,H,W,3), out_alpha / out_T_final / out_last
+ * (B,H,W).  Each candidate keeps its own state, so candidate b's gradients
+ * are its own (the reference's batch backward reads the last candidate's
+ * cache for all, tile_rasterize.cu:533-628). */
+int dbmp_forward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
+                       const dbmp_bitmaps* bm, const dbmp_tile_lists* lists, float eps_T,
+                       float* out_color, float* out_alpha, float* out_T_final, int32_t* out_last,
+                       void* stream);
+
+/* Backward of dbmp_forward_batch: grad_color (B,H,W,3), grad_alpha (B,H,W)
+ * or NULL; gradients stacked (B,N,2), (B,N), (B,N), (B,N), (B,N,3). */
+int dbmp_backward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
+                        const dbmp_bitmaps* bm, const dbmp_tile_lists* lists, float eps_T,
+                        const float* grad_color, const float* grad_alpha, const float* out_color,
+                        const float* out_T_final, const int32_t* out_last, float* grad_means2D,
+                        float* grad_radii, float* grad_rotations, float* grad_opacities,
+                        float* grad_colors, void* stream);
+
 /* Backward; replaces _C.rasterize_tiles_backward_class (ext.cpp:31-50 ->
  * tile_rasterize.cu:638-780).  grad_color (H,W,3), grad_alpha (H,W) or NULL;
  * out_color / out_T_final / out_last are dbmp_forward's outputs for the same

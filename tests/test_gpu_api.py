@@ -63,27 +63,74 @@ def test_fp16_function_returns_fp16_and_accepts_fp32_params(dev):
     assert all(p.grad is not None and p.grad.dtype == torch.float32 for p in P)
 
 
-def test_batch_function_per_candidate_gradients(dev):
-    """Each candidate's gradient is its own (the reference's batch backward
-    reuses the last candidate's cache for all)."""
-    sc = S.make_scene(0, H=96, W=96, N=32, tsz=32, with_target=False)
+def _batch_case(sc, dev, B, seed, with_maps):
+    """B perturbed copies of a scene's parameters (and per-candidate colour
+    maps) through TileRasterizerBatchFunction; -> params, outputs, grads."""
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rng = np.random.default_rng(seed)
+    P = [np.stack([a + rng.normal(0, s, a.shape).astype(np.float32) for _ in range(B)])
+         for a, s in ((sc.means2D, 3.0), (sc.radii, 0.5), (sc.rotations, 0.3), (sc.opacities, 0.5),
+                      (sc.colors, 0.5))]
+    P[1] = np.abs(P[1]) + 1.0
+    leaves = [t(a).requires_grad_() for a in P]
+    T, Ht, Wt = sc.templates.shape
+    co = rng.uniform(0, 1, (B, sc.N, Ht, Wt, 3)).astype(np.float32) if with_maps else None
+    c_blend = 0.4 if with_maps else 0.0
+    rast = TR.TileRasterizer(sc.H, sc.W, 16, 0.0, 1.0, 151, sc.N)
+    rast.eps_T = 0.0
+    lr = torch.zeros(7, device=dev)
+    cb, ab = rast.forward_batch(*leaves, t(co) if with_maps else torch.empty(0, device=dev),
+                                t(sc.templates), t(sc.bmp_sel), c_blend, lr)
+    gC = rng.standard_normal((B, sc.H, sc.W, 3)).astype(np.float32)
+    gA = rng.standard_normal((B, sc.H, sc.W)).astype(np.float32)
+    ((cb * t(gC)).sum() + (ab * t(gA)).sum()).backward()
+    return P, co, c_blend, cb.detach().cpu().numpy(), ab.detach().cpu().numpy(), gC, gA, leaves
+
+
+@pytest.mark.parametrize("with_maps", [False, True])
+def test_batch_function_matches_oracle_per_candidate(dev, with_maps):
+    """Every candidate of one batched launch equals the oracle on that
+    candidate's own parameters (image and gradients); the reference's batch
+    backward reuses the last candidate's cache for all (D3)."""
+    sc = S.make_scene(0, H=96, W=112, N=40, tsz=32, with_target=False)
+    B = 3
+    P, co, cbl, cb, ab, gC, gA, leaves = _batch_case(sc, dev, B, 4, with_maps)
+    names = ("means2D", "radii", "rotations", "opacities", "colors")
+    for b in range(B):
+        import copy
+        s2 = copy.copy(sc)
+        for k, a in zip(names, P):
+            setattr(s2, k, a[b])
+        s2.c_blend = cbl
+        offs, idx = O.bin_scene(s2)
+        osc = O.scene_from(s2, offs, idx, colors_orig=co[b] if with_maps else None)
+        rc, ra, _ = O.forward(osc)
+        e = max(np.abs(cb[b] - rc).max(), np.abs(ab[b] - ra).max())
+        ref = O.backward(osc, gC[b].astype(np.float64), gA[b].astype(np.float64))
+        got = {k: leaf.grad[b].cpu().numpy() for k, leaf in zip(names, leaves)}
+        err = normwise(got, ref)
+        print(f"candidate {b}: image {e:.2e}, gradients {max(err.values()):.2e}")
+        assert e <= 1e-5 and max(err.values()) <= 1e-4, (b, e, err)
+
+
+def test_batch_function_beyond_one_launch(dev):
+    """35 candidates: two launches of at most 32; each candidate equals its
+    single-candidate render and gradients."""
+    sc = S.make_scene(0, H=48, W=64, N=10, tsz=16, with_target=False)
+    B = 35
+    P, _, _, cb, ab, gC, gA, leaves = _batch_case(sc, dev, B, 9, False)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     rast = TR.TileRasterizer(sc.H, sc.W, 16, 0.0, 1.0, 151, sc.N)
-    B = 3
-    rng = np.random.default_rng(4)
-    means = np.stack([sc.means2D + rng.normal(0, 3, sc.means2D.shape).astype(np.float32) for _ in range(B)])
-    batch = [t(means).requires_grad_()] + [t(np.stack([a] * B)).requires_grad_()
-                                           for a in (sc.radii, sc.rotations, sc.opacities, sc.colors)]
+    rast.eps_T = 0.0
     lr = torch.zeros(7, device=dev)
-    cb, ab = rast.forward_batch(*batch, torch.empty(0, device=dev), t(sc.templates), t(sc.bmp_sel), 0.0, lr)
-    (cb.sum() + ab.sum()).backward()
-    for b in range(B):
-        single = [t(means[b]).requires_grad_()] + _leaves(sc, dev)[1:]
+    for b in (0, 17, 31, 32, 34):
+        single = [t(a[b]).requires_grad_() for a in P]
         cs, as_ = rast(*single, torch.empty(0, device=dev), t(sc.templates), t(sc.bmp_sel), 0.0, lr)
-        (cs.sum() + as_.sum()).backward()
-        assert torch.equal(cs, cb[b].detach())
-        for pb, ps in zip(batch, single):
-            assert torch.allclose(pb.grad[b], ps.grad, rtol=1e-5, atol=1e-6)
+        ((cs * t(gC[b])).sum() + (as_ * t(gA[b])).sum()).backward()
+        assert np.array_equal(cs.detach().cpu().numpy(), cb[b]), b
+        assert np.array_equal(as_.detach().cpu().numpy(), ab[b]), b
+        for pb, ps in zip(leaves, single):
+            assert torch.allclose(pb.grad[b], ps.grad, rtol=1e-5, atol=1e-6), b
 
 
 def test_errors_raise(dev):
