@@ -13,8 +13,9 @@ device synchronisations); per-step hipEvent times of the fwd+bwd segment
 ``roofline``: the dominant kernel (backward raster) -- its compulsory HBM
 bytes per launch divided by its mean duration, measured live with hipEvents
 on the launch stream, against 8 TB/s.  ``cpu_baseline``: the reference
-algorithm in fp32 on the host cores (OpenMP build of the oracle), timed on a
-bounded sample of tiles and scaled to the full canvas.
+algorithm in fp32 on the host cores (OpenMP build of the oracle): full
+optimisation steps of the same workload (median of 3), and the plain
+workload's 100-step fit (``plain_fit``, also run on the GPU).
 """
 from __future__ import annotations
 
@@ -51,7 +52,8 @@ def parse():
                          "Adam as HIP kernels); autograd: the reference-compatible TileRasterizer "
                          "behind torch autograd and torch.optim.Adam")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-tiles", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="full host steps of the scale workload timed for the CPU baseline")
     ap.add_argument("--psnr-steps", type=int, default=0,
                     help="also fit this many Adam steps at fp32 and at --precision from the same "
                          "start and report both final PSNRs")
@@ -86,35 +88,121 @@ def algorithmic_bytes(H, W, N, T, Ht, Wt, P, tiles, b_p, b_t):
     return b_bin, b_fwd, b_bwd
 
 
-def cpu_baseline(scene, ntiles_sample):
-    """The reference algorithm in fp32 (per-pixel caches, O(K^2) backward) on
-    the host's cores, on a random sample of tiles, scaled to the full canvas."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+class _CpuFit:
+    """The reference's optimisation step on the host: binning, forward and
+    backward of the fp32 port of the reference algorithm (per-pixel caches,
+    O(K^2) suffix loop, all host cores), background composite + MSE and
+    torch.optim.Adam's update with the reference's clamp in numpy."""
+
+    def __init__(self, scene, target, lr=1e-2, betas=(0.9, 0.999), eps=1e-8):
+        import oracle as O
+        self.O, self.sc, self.target = O, scene, target.astype(np.float32)
+        self.p = [np.array(a, np.float32) for a in (scene.means2D, scene.radii, scene.rotations,
+                                                     scene.opacities, scene.colors)]
+        self.m = [np.zeros_like(a) for a in self.p]
+        self.v = [np.zeros_like(a) for a in self.p]
+        self.lr, self.b1, self.b2, self.eps, self.t = lr, betas[0], betas[1], eps, 0
+        self.ext = O.template_extents_np(scene.templates)
+
+    def step(self) -> float:
+        O, sc = self.O, self.sc
+        m2, r, th, v, c = self.p
+        offs, idx = O.bin_lists_c(m2, r, th, sc.bmp_sel, self.ext, sc.H, sc.W, sc.tile_size)
+        osc = O.OracleScene(sc.H, sc.W, m2, r, th, v, c, sc.templates, sc.bmp_sel, offs, idx,
+                            ts=sc.tile_size, alpha_ub=sc.alpha_ub)
+        col, al, _ = O.forward(osc, "f32")
+        res = col + (1.0 - al)[..., None] - self.target
+        loss = float((res.astype(np.float64) ** 2).mean())
+        gC = (2.0 / res.size) * res
+        gA = -gC.sum(-1)
+        g = O.backward(osc, gC, gA, "f32")
+        self.t += 1
+        bc1, bc2 = 1 - self.b1 ** self.t, 1 - self.b2 ** self.t
+        for i, k in enumerate(("means2D", "radii", "rotations", "opacities", "colors")):
+            gi = g[k].astype(np.float32)
+            self.m[i] = self.b1 * self.m[i] + (1 - self.b1) * gi
+            self.v[i] = self.b2 * self.v[i] + (1 - self.b2) * gi * gi
+            self.p[i] = (self.p[i] - (self.lr / bc1) * self.m[i] /
+                         (np.sqrt(self.v[i]) / math.sqrt(bc2) + self.eps)).astype(np.float32)
+        self.p[0][:, 0] = np.clip(self.p[0][:, 0], 0, sc.W - 1)
+        self.p[0][:, 1] = np.clip(self.p[0][:, 1], 0, sc.H - 1)
+        self.p[1] = np.clip(self.p[1], 2, min(sc.H, sc.W) // 4).astype(np.float32)
+        return loss
+
+
+def cpu_baseline(scene, scale_steps=3, c0_steps=100):
+    """The reference algorithm in fp32 on the host's cores: ``scale_steps``
+    full optimisation steps of the scale workload (median step time is the
+    value), and the plain workload's ``c0_steps``-step fit."""
     import oracle as O
-    L = O.lib("f32")
-    cores = int(L.oracle_num_threads())
+    from diffbmp_amd import scenes as S
+    cores = int(O.lib("f32").oracle_num_threads())
+    fit = _CpuFit(scene, scene.target)
+    times = []
+    for _ in range(scale_steps):
+        t0 = time.perf_counter()
+        fit.step()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    # plain workload: 100 Adam steps to the second random render (the target
+    # rendered by the same port)
+    c0 = S.make_scene(0, with_target=False)
+    tp = S.make_target_params(0, c0.H, c0.W, c0.N)
+    offs, idx = O.bin_lists_c(tp[0], tp[1], tp[2], c0.bmp_sel, O.template_extents_np(c0.templates),
+                              c0.H, c0.W, c0.tile_size)
+    col, al, _ = O.forward(O.OracleScene(c0.H, c0.W, *tp, c0.templates, c0.bmp_sel, offs, idx), "f32")
+    f0 = _CpuFit(c0, col + (1.0 - al)[..., None])
     t0 = time.perf_counter()
-    ext = O.template_extents_np(scene.templates)
-    offs, idx = O.bin_lists_c(scene.means2D, scene.radii, scene.rotations, scene.bmp_sel, ext,
-                              scene.H, scene.W, scene.tile_size)
-    t_bin = time.perf_counter() - t0
-    tiles = len(offs) - 1
-    rng = np.random.default_rng(99)
-    sub = np.sort(rng.choice(tiles, min(ntiles_sample, tiles), replace=False)).astype(np.int32)
-    osc = O.scene_from(scene, offs, idx, tile_subset=sub)
-    gC = np.full((scene.H, scene.W, 3), 1e-6, np.float32)
-    gA = np.full((scene.H, scene.W), 1e-6, np.float32)
+    losses = [f0.step() for _ in range(c0_steps)]
+    c0_s = time.perf_counter() - t0
+    return dict(value=round(1.0 / med, 5), unit="iters/s", cores=cores, kind="port",
+                cpu_model=_cpu_model(),
+                sample=f"scale workload: {scale_steps} full optimisation steps (binning, forward, "
+                       f"composite/MSE, backward, Adam; fp32 port of the reference algorithm), "
+                       f"median {med * 1e3:.0f} ms per step",
+                step_ms=[round(x * 1e3, 1) for x in times],
+                plain_fit={"steps": c0_steps, "steps_per_s": round(c0_steps / c0_s, 3),
+                           "loss_first": round(losses[0], 6), "loss_last": round(losses[-1], 6)})
+
+
+def gpu_plain_fit(dev, steps=100):
+    """The plain workload end to end on the GPU: FitJob, ``steps`` Adam steps
+    (lr 1e-2) of MSE to the second random render (rendered by the drop-in
+    rasterizer)."""
+    import torch
+    from diffbmp_amd import scenes as S
+    from diffbmp_amd.fit import FitJob
+    from diffbmp_amd.tile_rasterizer import TileRasterizer
+    sc = S.make_scene(0, with_target=False)
+    tp = S.make_target_params(0, sc.H, sc.W, sc.N)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rast = TileRasterizer(sc.H, sc.W, 16, 0.0, 1.0, 151, sc.N)
+    with torch.no_grad():
+        c, a = rast(*[t(x) for x in tp], torch.empty(0, device=dev), t(sc.templates), t(sc.bmp_sel),
+                    0.0, torch.zeros(7, device=dev))
+    target = (c + (1 - a).unsqueeze(-1)).contiguous()
+    init = [t(x) for x in (sc.means2D, sc.radii, sc.rotations, sc.opacities, sc.colors)]
+    job = FitJob(*init, t(sc.templates), t(sc.bmp_sel), target, lr=1e-2, max_steps=steps)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    O.forward(osc, "f32")
-    t_f = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.backward(osc, gC, gA, "f32")
-    t_b = time.perf_counter() - t0
-    scale = tiles / len(sub)
-    step_s = t_bin + (t_f + t_b) * scale
-    return dict(value=1.0 / step_s, unit="fwd+bwd iters/s", cores=cores, kind="port",
-                sample=f"{len(sub)} of {tiles} tiles (fwd+bwd, fp32 port of the reference "
-                       f"algorithm) scaled x{scale:.1f}, plus full binning; "
-                       f"{step_s * 1e3:.0f} ms per full step")
+    for _ in range(steps):
+        job.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    L = job.loss_history().cpu().numpy()
+    return {"steps": steps, "steps_per_s": round(steps / el, 2), "loss_first": round(float(L[0]), 6),
+            "loss_last": round(float(L[-1]), 6)}
 
 
 def make_problem(sc, dev, sdt):
@@ -520,8 +608,9 @@ def main():
                                      "source": "profiles/pmc_valu_scale_fp32.json"}
     if args.psnr_steps:
         out["psnr"] = psnr_compare(args, sc, dev, sdt)
+    out["plain_fit"] = gpu_plain_fit(dev)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_tiles)
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
     print(json.dumps(out))
 
 

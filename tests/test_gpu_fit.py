@@ -163,3 +163,61 @@ def test_fitjob_reduced_precision_runs(dev, store):
     assert np.all(np.isfinite(l)) and l[-1] < l[0]
     for k in GROUPS:
         assert torch.equal(job.store[k], job.params[k].to(store))
+
+
+def c0_target(dev):
+    """The plain workload's target: the render over white of a second random
+    256-primitive parameter set (role-1 seed) through the drop-in rasterizer."""
+    sc = S.make_scene(0, with_target=False)
+    tp = S.make_target_params(0, sc.H, sc.W, sc.N)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rast = TileRasterizer(sc.H, sc.W, 16, 0.0, 1.0, 151, sc.N)
+    with torch.no_grad():
+        c, a = rast(*[t(x) for x in tp], torch.empty(0, device=dev), t(sc.templates), t(sc.bmp_sel),
+                    0.0, torch.zeros(7, device=dev))
+    return sc, (c + (1 - a).unsqueeze(-1)).contiguous()
+
+
+def test_c0_end_to_end_fit(dev):
+    """The plain workload end to end: 100 Adam steps (lr 1e-2) of MSE to the
+    second random render, as the device-resident FitJob step and as the
+    reference loop (drop-in TileRasterizer under torch autograd,
+    torch.optim.Adam, the reference's clamp), step by step."""
+    sc, target = c0_target(dev)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    init = [t(a) for a in (sc.means2D, sc.radii, sc.rotations, sc.opacities, sc.colors)]
+    steps = 100
+    job = FitJob(*init, t(sc.templates), t(sc.bmp_sel), target, lr=1e-2, eps_T=1e-6, max_steps=steps)
+    rast = TileRasterizer(sc.H, sc.W, 16, 0.0, 1.0, 151, sc.N)
+    leaves = [a.clone().requires_grad_() for a in init]
+    opt = torch.optim.Adam(leaves, lr=1e-2)
+    empty, lr_conf = torch.empty(0, device=dev), torch.zeros(7, device=dev)
+    ref = []
+    for _ in range(steps):
+        job.step()
+        opt.zero_grad()
+        c, a = rast(*leaves, empty, t(sc.templates), t(sc.bmp_sel), 0.0, lr_conf)
+        loss = F.mse_loss(c + (1 - a).unsqueeze(-1), target)
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            leaves[0][:, 0].clamp_(0, sc.W - 1)
+            leaves[0][:, 1].clamp_(0, sc.H - 1)
+            leaves[1].clamp_(2, sc.W // 4)
+        ref.append(loss.item())
+    got = job.loss_history().cpu().numpy()
+    ref = np.array(ref)
+    dev_rel = np.abs(got - ref) / ref
+    print(f"loss {ref[0]:.5f} -> {ref[-1]:.5f}; max relative deviation {dev_rel.max():.2e}")
+    assert ref[-1] < 0.6 * ref[0]
+    assert dev_rel.max() <= 1e-3, dev_rel.max()
+    # Adam steps a scalar by about lr whatever the size of its gradient, so a
+    # scalar whose gradient sits at rounding level may walk either way: the
+    # parameters agree in the bulk, a few drift by up to steps * lr.  (The
+    # plain workload's template is a radially symmetric disk: every rotation
+    # gradient is rounding noise, and the rotations random-walk in both loops.)
+    for k, leaf in zip(GROUPS, leaves):
+        d = (job.params[k] - leaf.detach()).abs().flatten()
+        med, far = d.median().item(), (d > 0.25).float().mean().item()
+        print(f"{k}: median |diff| {med:.2e}, fraction > 0.25: {far:.4f}, max {d.max().item():.3f}")
+        assert med <= 5e-3 and far <= 0.02, k
