@@ -187,6 +187,8 @@ EXPORTS = {
                                            ctypes.c_int64, ctypes.c_void_p]),
     "dbmp_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "dbmp_batch_pair_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "dbmp_batch_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
     "dbmp_psd_bounds": (ctypes.c_int, [ctypes.POINTER(LayerPrims), ctypes.c_int, ctypes.c_int,
                                        ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -59,6 +59,7 @@ struct dbmp_batch {
   long long* host_total_dev = nullptr;  // its device address
   long long cull_cap = 0;
   float4* cull = nullptr;               // per-template content boxes of every job
+  std::vector<int> h_prim_start;        // host copy of prim_start (J+1)
   // graph of the current set
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -463,6 +464,7 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
   }
   HIP_TRY(hipEventRecord(b->upload_done, st));
   b->upload_pending = true;
+  b->h_prim_start = ps;
   b->J = J;
   b->Ntot = (int)Nt;
   b->tiles_tot = (int)Tt;
@@ -496,6 +498,16 @@ int dbmp_batch_step(dbmp_batch* b, int32_t use_graph, void* stream) {
     }
   }
   HIP_TRY(hipGraphLaunch(b->exec, st));
+  return 0;
+}
+
+int dbmp_batch_gradients(dbmp_batch* b, int32_t job, float* rows, void* stream) {
+  if (!b || !rows) return fail(DBMP_E_ARG, "NULL argument");
+  if (job < 0 || job >= b->J) return fail(DBMP_E_ARG, "job %d not in the set of %d", job, b->J);
+  const int n0 = b->h_prim_start[job], n1 = b->h_prim_start[job + 1];
+  if (n1 > n0)
+    HIP_TRY(hipMemcpyAsync(rows, b->grad8 + 8LL * n0, sizeof(float) * 8LL * (n1 - n0),
+                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return 0;
 }
 

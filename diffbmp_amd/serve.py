@@ -53,6 +53,7 @@ class JobSpec:
     name: str = ""
     template_size: int = 256
     library: str = "stress"           # template_library kind
+    clamp: bool = True                # the reference's clamp of x, y, r after each step
 
 
 def job_bytes(spec: JobSpec, store_dtype=torch.float16) -> int:
@@ -223,7 +224,7 @@ class FitState:
             j.lr[i] = float(s.lr)
         j.alpha_upper_bound = float(s.alpha_upper_bound)
         j.eps_T = float(s.eps_T)
-        j.clamp = 1
+        j.clamp = int(s.clamp)
         j.steps_done = self.steps_done
         return j
 
@@ -297,6 +298,14 @@ class BatchEngine:
         check(self._L.dbmp_batch_step(self._b, int(self.use_graph), _native._stream(self.device)))
         for j in self.jobs:
             j.steps_done += 1
+
+    def gradients(self, job: FitState) -> torch.Tensor:
+        """(N,8) gradient rows of ``job`` from the most recent step (x, y, r,
+        theta, opacity logit, 3 colour logits), before its Adam update."""
+        out = torch.empty((job.spec.num_primitives, 8), dtype=torch.float32, device=self.device)
+        check(self._L.dbmp_batch_gradients(self._b, self.jobs.index(job), _native._ptr(out),
+                                           _native._stream(self.device)))
+        return out
 
     def pair_count(self) -> int:
         """Pairs of the most recent completed step (host-mapped mirror)."""

@@ -364,6 +364,12 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
  * per job set into a hipGraph and replay it. */
 int dbmp_batch_step(dbmp_batch* b, int32_t use_graph, void* stream);
 
+/* The gradient rows (N_j,8) of job `job` (its index in the last
+ * dbmp_batch_set_jobs) from the batch's most recent step, copied on `stream`
+ * into `rows`: dL/d(x, y, r, theta, opacity logit, 3 colour logits) of that
+ * step's MSE at the parameters it rendered, before its Adam update. */
+int dbmp_batch_gradients(dbmp_batch* b, int32_t job, float* rows, void* stream);
+
 /* Pair count of the most recent step that has completed on the device. */
 int dbmp_batch_pair_count(dbmp_batch* b, int64_t* pairs);
 
