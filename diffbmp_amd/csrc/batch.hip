@@ -59,6 +59,9 @@ struct dbmp_batch {
   long long* host_total_dev = nullptr;  // its device address
   long long cull_cap = 0;
   float4* cull = nullptr;               // per-template content boxes of every job
+  int* counters = nullptr;              // 2 x kXcds work counters (forward, backward)
+  int cus = 0;                          // compute units of the device
+  int occ_fwd = 0, occ_bwd = 0;         // resident workgroups per CU of the raster kernels
   std::vector<int> h_prim_start;        // host copy of prim_start (J+1)
   // graph of the current set
   hipGraph_t graph = nullptr;
@@ -66,6 +69,15 @@ struct dbmp_batch {
 };
 
 namespace {
+
+// Binning of the batched step: lists per super-tile of 16 << kBatchBinShift
+// px (each primitive's 16-px tile range filters it per 16x16 block), so the
+// sort handles 16x fewer pairs than per-tile lists at 4.
+#ifndef DBMP_BATCH_BIN_SHIFT
+#define DBMP_BATCH_BIN_SHIFT 2
+#endif
+constexpr int kBatchBinShift = DBMP_BATCH_BIN_SHIFT;
+constexpr int kBatchBin = 1 << kBatchBinShift;
 
 int* prim_start(dbmp_batch* b) { return b->dstarts; }
 int* tile_start(dbmp_batch* b) { return b->dstarts + (b->job_cap + 1); }
@@ -137,10 +149,35 @@ int ensure_jobs(dbmp_batch* b, int J) {
   return 0;
 }
 
+// Resident workgroups per CU of the batched raster kernels of storage type DT.
+template <int DT>
+int raster_occupancy(dbmp_batch* b) {
+  if (b->occ_fwd > 0) return 0;
+  HIP_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  int f = 0, g = 0;
+  if (b->blend) {
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, k_batch_fwd<DT, true>, kThreads, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g, k_batch_bwd<DT, true>, kThreads, 0));
+  } else {
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&f, k_batch_fwd<DT, false>, kThreads, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g, k_batch_bwd<DT, false>, kThreads, 0));
+  }
+  b->occ_fwd = std::max(f, 1);
+  b->occ_bwd = std::max(g, 1);
+  return 0;
+}
+
 template <int DT>
 int launch_step(dbmp_batch* b, hipStream_t st) {
   const int J = b->J, N = b->Ntot;
   const long long cap = b->pair_cap;
+  const int gf = (int)std::min<long long>(b->blocks_tot, (long long)b->cus * b->occ_fwd);
+  const int gb = (int)std::min<long long>(b->blocks_tot, (long long)b->cus * b->occ_bwd);
+  const WorkQueue qf{b->counters, b->blocks_tot};
+  const WorkQueue qb{b->counters + kXcds * kCounterStride, b->blocks_tot};
+  hipLaunchKernelGGL(k_batch_reset, dim3(ceil_div(2 * kXcds * kCounterStride, 256)), dim3(256), 0, st,
+                     b->counters, 2 * kXcds * kCounterStride);
+  LAUNCH_CHECK("k_batch_reset");
   if (N > 0) {
     hipLaunchKernelGGL(k_batch_prep<DT>, dim3(ceil_div(N, 256)), dim3(256), 0, st, b->djobs,
                        prim_start(b), J, b->recs, b->counts, b->ranges);
@@ -182,22 +219,22 @@ int launch_step(dbmp_batch* b, hipStream_t st) {
                      dim3(256), 0, st, kcur, b->dtot, cap, b->tiles_tot, b->toffs);
   LAUNCH_CHECK("k_tile_bounds");
   if (b->blend)
-    hipLaunchKernelGGL((k_batch_fwd<DT, true>), dim3(b->blocks_tot), dim3(kThreads), 0, st, b->djobs,
-                       block_start(b), J);
+    hipLaunchKernelGGL((k_batch_fwd<DT, true>), dim3(gf), dim3(kThreads), 0, st, b->djobs,
+                       block_start(b), J, qf);
   else
-    hipLaunchKernelGGL((k_batch_fwd<DT, false>), dim3(b->blocks_tot), dim3(kThreads), 0, st,
-                       b->djobs, block_start(b), J);
+    hipLaunchKernelGGL((k_batch_fwd<DT, false>), dim3(gf), dim3(kThreads), 0, st, b->djobs,
+                       block_start(b), J, qf);
   LAUNCH_CHECK("k_batch_fwd");
   if (N > 0) {
     hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, b->grad8, 8LL * N, 0.f);
     LAUNCH_CHECK("k_fill_f32");
   }
   if (b->blend)
-    hipLaunchKernelGGL((k_batch_bwd<DT, true>), dim3(b->blocks_tot), dim3(kThreads), 0, st, b->djobs,
-                       b->dstate, block_start(b), prim_start(b), J, b->grad8);
+    hipLaunchKernelGGL((k_batch_bwd<DT, true>), dim3(gb), dim3(kThreads), 0, st, b->djobs,
+                       b->dstate, block_start(b), prim_start(b), J, b->grad8, qb);
   else
-    hipLaunchKernelGGL((k_batch_bwd<DT, false>), dim3(b->blocks_tot), dim3(kThreads), 0, st,
-                       b->djobs, b->dstate, block_start(b), prim_start(b), J, b->grad8);
+    hipLaunchKernelGGL((k_batch_bwd<DT, false>), dim3(gb), dim3(kThreads), 0, st, b->djobs,
+                       b->dstate, block_start(b), prim_start(b), J, b->grad8, qb);
   LAUNCH_CHECK("k_batch_bwd");
   hipLaunchKernelGGL(k_batch_tick, dim3(ceil_div(J, 64)), dim3(64), 0, st, b->djobs, b->dstate, J,
                      b->beta1, b->beta2, (const long long*)b->dtot,
@@ -209,6 +246,12 @@ int launch_step(dbmp_batch* b, hipStream_t st) {
     LAUNCH_CHECK("k_batch_adam");
   }
   return 0;
+}
+
+int raster_occupancy_dt(dbmp_batch* b) {
+  if (b->dtype == DBMP_F32) return raster_occupancy<kF32>(b);
+  if (b->dtype == DBMP_F16) return raster_occupancy<kF16>(b);
+  return raster_occupancy<kBF16>(b);
 }
 
 int launch_step_dt(dbmp_batch* b, hipStream_t st) {
@@ -239,6 +282,7 @@ int dbmp_batch_create(dbmp_batch** out, int32_t storage_dtype, float beta1, floa
     rc = fail(DBMP_E_HIP, "hipEventCreate failed");
   if (!rc) rc = grow(&b->dtot, 1);
   if (!rc) rc = grow(&b->totals, 256);
+  if (!rc) rc = grow(&b->counters, 2 * kXcds * kCounterStride);
   if (!rc && hipHostMalloc((void**)&b->host_total, sizeof(long long), hipHostMallocMapped) != hipSuccess)
     rc = fail(DBMP_E_ALLOC, "hipHostMalloc failed");
   if (!rc && hipHostGetDevicePointer((void**)&b->host_total_dev, b->host_total, 0) != hipSuccess)
@@ -259,7 +303,7 @@ int dbmp_batch_destroy(dbmp_batch* b) {
   if (b->cap_stream) (void)hipStreamSynchronize(b->cap_stream);
   void* ptrs[] = {b->djobs, b->dstate, b->dstarts, b->recs, b->counts, b->offs, b->ranges,
                   b->grad8, b->keysA, b->keysB, b->valsB, b->tidx, b->hist, b->totals, b->toffs,
-                  b->dtot, b->cull};
+                  b->dtot, b->cull, b->counters};
   (void)hipDeviceSynchronize();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -319,9 +363,10 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     ts[i] = (int)Tt;
     bs[i] = (int)Bt;
     const long long tx = (jobs[i].width + kSub - 1) / kSub, ty = (jobs[i].height + kSub - 1) / kSub;
+    const long long lx = (tx + kBatchBin - 1) / kBatchBin, ly = (ty + kBatchBin - 1) / kBatchBin;
     Nt += jobs[i].num_primitives;
-    Tt += tx * ty;
-    Bt += tx * ty;
+    Tt += lx * ly;                          // lists: one per super-tile
+    Bt += lx * ly * kBatchBin * kBatchBin;  // 16x16 blocks of the super-tiles
   }
   ps[J] = (int)Nt;
   ts[J] = (int)Tt;
@@ -373,10 +418,10 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.hsy = 0.5f * (float)(s.tmpl_height - 1);
     A.H = s.height;
     A.W = s.width;
-    A.ts = kSub;
-    A.tilesX = (s.width + kSub - 1) / kSub;
+    A.ts = kSub * kBatchBin;
+    A.tilesX = ((s.width + kSub - 1) / kSub + kBatchBin - 1) / kBatchBin;
     A.tiles = ts[i + 1] - ts[i];
-    A.subs = 1;
+    A.subs = kBatchBin * kBatchBin;
     A.eps = s.eps_T;
     // the job's binning ranges: after a pair overflow every pixel walks its
     // job's primitives, each kept to the tiles of its range
@@ -385,7 +430,7 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.cost = nullptr;
     A.cull = b->cull + cs[i];   // filled from the job's extents after the upload
     A.ntmpl = s.num_templates;
-    A.bin_shift = 0;
+    A.bin_shift = kBatchBinShift;
     d.alpha_ub = s.alpha_upper_bound;
     d.L.target = s.target;
     d.L.bg = s.background;
@@ -400,7 +445,8 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     d.sel = s.bmp_sel;
     d.ext = s.ext;
     d.T = s.num_templates;
-    d.tilesY = (s.height + kSub - 1) / kSub;
+    d.tilesX16 = (s.width + kSub - 1) / kSub;
+    d.tilesY16 = (s.height + kSub - 1) / kSub;
     d.color = s.out_color;
     d.alpha = s.out_alpha;
     d.Tfin = s.out_T;
@@ -470,7 +516,9 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
   b->tiles_tot = (int)Tt;
   b->blocks_tot = (int)Bt;
   b->pair_cap = pair_capacity;
+  if (b->blend != blend) b->occ_fwd = b->occ_bwd = 0;   // the other kernels' occupancy
   b->blend = blend;
+  if ((rc = raster_occupancy_dt(b))) return rc;   // here, never under stream capture
   drop_graph(b);
   return 0;
 }

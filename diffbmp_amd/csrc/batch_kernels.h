@@ -36,7 +36,7 @@ struct BatchJob {
   const int* sel;      // (N) template index
   const float* ext;    // (T,4) template extents
   int T;
-  int tilesY;
+  int tilesX16, tilesY16;   // 16-px tiles of the canvas (the binning's ranges)
   float* color;        // (H,W,3) forward output, premultiplied over black
   float* alpha;        // (H,W)
   float* Tfin;         // (H,W) product of the nonzero (1 - a) factors
@@ -109,14 +109,14 @@ static __global__ __launch_bounds__(256) void k_batch_prep(const BatchJob* __res
   int4 rg;
   const RasterArgs& A = B.A;
   prep_core(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], B.sel[n], B.ext, B.T, A.Ht, A.Wt, A.H,
-            A.W, A.ts, A.tilesX, B.tilesY, B.alpha_ub, 0, p, cnt, rg);
+            A.W, kSub, B.tilesX16, B.tilesY16, B.alpha_ub, A.bin_shift, p, cnt, rg);
   recs[g] = p;
   counts[g] = cnt;
   ranges[g] = rg;
 }
 
-// One wave per primitive: key = job tile base + row-major local tile id,
-// value = local primitive id.
+// One wave per primitive: key = job list base + row-major local list id
+// (lists per super-tile of 16 << bin_shift px), value = local primitive id.
 static __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__ jobs,
                                                     const int* __restrict__ prim_start,
                                                     const int* __restrict__ tile_start, int J,
@@ -133,9 +133,10 @@ static __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __res
   if (cnt == 0) return;
   const int j = find_job(prim_start, J, g);
   const int n = g - prim_start[j];
-  const int tilesX = jobs[j].A.tilesX, tb = tile_start[j];
+  const int tilesX = jobs[j].A.tilesX, tb = tile_start[j], sh = jobs[j].A.bin_shift;
   const long long off = offs[g];
-  const int4 rg = ranges[g];
+  const int4 r16 = ranges[g];
+  const int4 rg = make_int4(r16.x >> sh, r16.y >> sh, r16.z >> sh, r16.w >> sh);
   const int nx = rg.y - rg.x + 1;
   for (int i = lane_id(); i < cnt; i += 64) {
     const int qy = i / nx, qx = i - qy * nx;
@@ -144,15 +145,60 @@ static __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __res
   }
 }
 
+// Persistent raster launches.  A launch has as many workgroups as fit on the
+// device at once; each takes (job, block) work items until none is left.
+// The items are cut into 8 contiguous bands of the concatenated blocks (jobs
+// in order, tiles row-major), one per XCD: workgroup w runs on XCD w mod 8
+// (workgroups are dealt round-robin to the XCDs, and all of them are resident
+// from the start), takes items of its own band from that band's counter --
+// neighbouring tiles, whose primitives and texels share its L2 -- and, once
+// its band is drained, items of the other bands.  Every workgroup leaves when
+// all eight counters have passed their band's end.  The counters are zeroed
+// by k_batch_reset before each step.
+constexpr int kXcds = 8;
+constexpr int kCounterStride = 32;   // ints: one 128-byte line per counter
+struct WorkQueue {
+  int* ctr;   // kXcds counters, kCounterStride apart
+  int total;  // items
+};
+
+__device__ __forceinline__ int next_item(const WorkQueue& q, int* s_item) {
+  if (threadIdx.x == 0) {
+    const int x = blockIdx.x & (kXcds - 1);
+    int item = -1;
+    for (int k = 0; k < kXcds && item < 0; ++k) {
+      const int band = (x + k) & (kXcds - 1);
+      const int b0 = (int)((long long)q.total * band / kXcds);
+      const int b1 = (int)((long long)q.total * (band + 1) / kXcds);
+      if (b1 <= b0) continue;
+      const int i = atomicAdd(q.ctr + band * kCounterStride, 1);
+      if (i < b1 - b0) item = b0 + i;
+    }
+    *s_item = item;
+  }
+  __syncthreads();
+  const int item = *s_item;
+  __syncthreads();   // every wave has read it before thread 0 writes the next
+  return item;
+}
+
+static __global__ void k_batch_reset(int* __restrict__ ctr, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ctr[i] = 0;
+}
+
 template <int DT, bool BLEND>
 static __global__ __launch_bounds__(kThreads) void k_batch_fwd(const BatchJob* __restrict__ jobs,
                                                         const int* __restrict__ block_start,
-                                                        int J) {
-  const int nb = block_start[J];
-  const int b = xcd_remap(blockIdx.x, nb);
-  const int j = find_job(block_start, J, b);
-  const BatchJob& B = jobs[j];
-  raster_fwd_block<DT, BLEND>(B.A, b - block_start[j], B.color, B.alpha, B.Tfin, B.last);
+                                                        int J, WorkQueue q) {
+  __shared__ int s_item;
+  for (;;) {
+    const int b = next_item(q, &s_item);
+    if (b < 0) break;
+    const int j = find_job(block_start, J, b);
+    const BatchJob& B = jobs[j];
+    raster_fwd_block<DT, BLEND>(B.A, b - block_start[j], B.color, B.alpha, B.Tfin, B.last);
+  }
 }
 
 template <int DT, bool BLEND>
@@ -160,15 +206,19 @@ static __global__ __launch_bounds__(kThreads) void k_batch_bwd(const BatchJob* _
                                                         const BatchState* __restrict__ state,
                                                         const int* __restrict__ block_start,
                                                         const int* __restrict__ prim_start,
-                                                        int J, float* __restrict__ grad8) {
-  const int nb = block_start[J];
-  const int b = xcd_remap(blockIdx.x, nb);
-  const int j = find_job(block_start, J, b);
-  const BatchJob& B = jobs[j];
-  LossArgs L = B.L;
-  L.loss = B.loss_hist + min(state[j].step, B.loss_cap - 1);
-  raster_bwd_block<DT, BLEND, true>(B.A, b - block_start[j], nullptr, nullptr, L, B.color, B.Tfin,
-                                    B.last, grad8 + 8L * prim_start[j]);
+                                                        int J, float* __restrict__ grad8,
+                                                        WorkQueue q) {
+  __shared__ int s_item;
+  for (;;) {
+    const int b = next_item(q, &s_item);
+    if (b < 0) break;
+    const int j = find_job(block_start, J, b);
+    const BatchJob& B = jobs[j];
+    LossArgs L = B.L;
+    L.loss = B.loss_hist + min(state[j].step, B.loss_cap - 1);
+    raster_bwd_block<DT, BLEND, true>(B.A, b - block_start[j], nullptr, nullptr, L, B.color, B.Tfin,
+                                      B.last, grad8 + 8L * prim_start[j]);
+  }
 }
 
 // t <- t + 1 and the bias corrections 1 - beta^t of step t
