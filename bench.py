@@ -61,6 +61,8 @@ def parse():
                     help="scale: the headline fit step; stress: the 2048^2 / 32768-primitive "
                          "RGBA colour-mapped fit step; serving: a Poisson stream of fit jobs "
                          "through the batched multi-job engine (each its own JSON line)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="leave the bounded serving run out of the headline line")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="arrival rate in jobs/s (0: every job arrives at t = 0, i.e. capacity)")
@@ -389,6 +391,29 @@ def run_stress(args, dev):
     print(json.dumps(out))
 
 
+def serving_summary(dev, jobs=40, rate=10.0, steps=200, seed=2025):
+    """A bounded serving run for the headline line: ``jobs`` fit jobs of the
+    serving mix arriving as a Poisson stream of ``rate`` jobs/s."""
+    import torch
+    from diffbmp_amd import serve as SV
+    SV.template_library(256)
+    rng = np.random.default_rng(seed)
+    specs = [SV.random_spec(rng, steps=steps) for _ in range(jobs)]
+    for s, a in zip(specs, SV.poisson_arrivals(rate, jobs, rng)):
+        s.arrival = float(a)
+    warm = SV.JobSpec(256, 256, 512, (0,), steps=5, seed=1, name="warm")
+    SV.Server(SV.BatchEngine(store_dtype=torch.float16, device=dev), SV.HbmBudget(1 << 34)).run([warm])
+    eng = SV.BatchEngine(store_dtype=torch.float16, device=dev)
+    stats = SV.Server(eng, SV.HbmBudget(64 << 30), max_jobs=32).run(specs)
+    lat = np.array([r.latency for r in stats.results])
+    return {"jobs": jobs, "arrival_rate_jobs_per_s": rate, "steps_per_job": steps,
+            "jobs_per_s": round(len(stats.results) / stats.wall, 3),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+            "workload": "serving mix: canvas side U{256,512,1024,2048}, primitives log-U[512,32768], "
+                        "1-16 RGBA 256^2 bitmaps, fp16 storage, 200 Adam steps per job"}
+
+
 def run_serving(args, dev):
     """Poisson stream of fit jobs (canvas side uniform over {256, 512, 1024,
     2048}, primitives log-uniform in [512, 32768], 1-16 of the stress
@@ -609,6 +634,8 @@ def main():
     if args.psnr_steps:
         out["psnr"] = psnr_compare(args, sc, dev, sdt)
     out["plain_fit"] = gpu_plain_fit(dev)
+    if not args.no_serving:
+        out["serving"] = serving_summary(dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
     print(json.dumps(out))
