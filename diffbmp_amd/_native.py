@@ -235,7 +235,14 @@ def _ptr(t: torch.Tensor | None):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(dev: torch.device):
+    """The current torch stream of ``dev`` as a hipStream_t."""
+    if _raw_stream is not None:
+        idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        return ctypes.c_void_p(_raw_stream(idx))
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
 
@@ -272,7 +279,9 @@ class Handle:
         # internal binning granularity: 64-px super-tiles for 16-px tiles
         # (same per-tile work, a sixteenth of the pairs to sort)
         self.bin_factor = 4 if self.ts == 16 else 1
-        self._pending = None          # (event, pinned count) of the last async binning
+        self._pending = None          # event of the last asynchronous pair-count read-back
+        self._readback = None         # pinned int64 the read-back lands in (reused)
+        self._readback_ev = None
         self._lazy_calls = 0
         # bumped by every call that recomputes the primitive records (a binning,
         # or a raster call without reuse_records): a caller that noted it after
@@ -418,8 +427,8 @@ class Handle:
         dev = ext.device
         N = prims.num_primitives
         self._lazy_calls += 1
-        if self._pending is not None and self._pending[0].query():
-            seen = int(self._pending[1][0])
+        if self._pending is not None and self._pending.query():
+            seen = int(self._readback[0])
             if seen > 0.9 * self.pair_capacity:
                 self.pair_capacity = int(seen * 1.3) + 4096
             self._pending = None
@@ -428,17 +437,22 @@ class Handle:
             self.pair_capacity = int(out[1].numel() * 1.3) + 4096
             return Lists(out[0], out[1], factor=factor, ranges=out[2] if factor > 1 else None)
         cap = self.pair_capacity
-        offs = torch.empty(self.list_count(factor) + 1, dtype=torch.int32, device=dev)
-        idx = torch.empty(cap, dtype=torch.int32, device=dev)
-        cnt = torch.empty(1, dtype=torch.int64, device=dev)
-        rg = self._ranges(prims, dev)
+        # one allocation: count (int64) | pad | ranges (N,4) | offsets | indices
+        L = self.list_count(factor) + 1
+        nr = 4 * max(N, 1)
+        buf = torch.empty(4 + nr + ((L + 3) & ~3) + cap, dtype=torch.int32, device=dev)
+        cnt = buf[0:2].view(torch.int64)
+        rg = buf[4:4 + nr].view(-1, 4)
+        offs = buf[4 + nr:4 + nr + L]
+        idx = buf[4 + nr + ((L + 3) & ~3):]
         self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
         if self._pending is None and self._lazy_calls % self.READBACK_EVERY == 0:
-            host = torch.empty(1, dtype=torch.int64, pin_memory=True)
-            host.copy_(cnt, non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(torch.cuda.current_stream(dev))
-            self._pending = (ev, host)
+            if self._readback is None:
+                self._readback = torch.empty(1, dtype=torch.int64, pin_memory=True)
+                self._readback_ev = torch.cuda.Event()
+            self._readback.copy_(cnt, non_blocking=True)
+            self._readback_ev.record(torch.cuda.current_stream(dev))
+            self._pending = self._readback_ev
         return Lists(offs, idx, cnt, cap, factor, rg)
 
     def forward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T: float, out_color, out_alpha,
