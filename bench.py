@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--psnr-steps", type=int, default=0,
                     help="also fit this many Adam steps at fp32 and at --precision from the same "
                          "start and report both final PSNRs")
-    ap.add_argument("--workload", default="scale", choices=["scale", "stress", "serving", "export"],
+    ap.add_argument("--workload", default="scale",
+                    choices=["scale", "stress", "serving", "export", "lanes"],
                     help="scale: the headline fit step; stress: the 2048^2 / 32768-primitive "
                          "RGBA colour-mapped fit step; serving: a Poisson stream of fit jobs "
                          "through the batched multi-job engine (each its own JSON line)")
@@ -474,6 +475,51 @@ def run_serving(args, dev):
     print(json.dumps(out))
 
 
+def run_lanes(args, dev):
+    """Lane utilisation of the raster kernels, from the profiling build
+    (libdiffbmp_amd_lanestats.so, loaded through DBMP_LIBRARY): per (wave,
+    list entry) evaluation of the forward and backward, the share with a
+    pushed lane and the pushed lanes per evaluation, over fused fit steps of
+    the scale and stress workloads."""
+    import ctypes
+    import torch
+    from diffbmp_amd import _native
+    from diffbmp_amd import scenes as S
+    from diffbmp_amd.fit import FitJob
+    if "lanestats" not in _native.LIB_PATH:
+        raise SystemExit("--workload lanes needs DBMP_LIBRARY=diffbmp_amd/libdiffbmp_amd_lanestats.so "
+                         "(python -m diffbmp_amd.build --lane-stats)")
+    L = _native.lib()
+    out = {"metric": "raster lane utilisation", "library": os.path.basename(_native.LIB_PATH),
+           "steps": args.steps}
+    for name, cfg, sdt, co in (("scale", 1, torch.float32, False), ("stress", 3, torch.float16, True)):
+        sc = S.make_scene(cfg)
+        pr = make_problem(sc, dev, sdt)
+        kw = {}
+        if co:
+            kw = dict(colors_orig=torch.from_numpy(np.ascontiguousarray(sc.colors_orig_tmpl)).to(dev),
+                      c_blend=sc.c_blend, colors_orig_per_template=True)
+        job = FitJob(*pr["params"], pr["templates"], pr["sel"], pr["target"], lr=1e-2, store_dtype=sdt,
+                     max_steps=args.warmup + args.steps + 4, **kw)
+        for _ in range(args.warmup):
+            job.step()
+        st = (ctypes.c_uint64 * 8)()
+        _native.check(L.dbmp_lane_stats(st, 1))
+        for _ in range(args.steps):
+            job.step()
+        _native.check(L.dbmp_lane_stats(st, 1))
+        res = {}
+        for k, b in (("forward", 0), ("backward", 3)):
+            ev, hit, lanes = st[b], st[b + 1], st[b + 2]
+            res[k] = {"evaluations_per_step": ev // args.steps,
+                      "share_with_a_pushed_lane": round(hit / max(ev, 1), 4),
+                      "pushed_lanes_per_evaluation": round(lanes / max(ev, 1), 2),
+                      "pushed_lanes_per_pushing_evaluation": round(lanes / max(hit, 1), 2),
+                      "lane_utilisation": round(lanes / max(64 * ev, 1), 4)}
+        out[name] = res
+    print(json.dumps(out))
+
+
 def run_export(args, dev):
     """Layer export of the scale scene: the uint8 RGBA crop (1.5 r square) of
     every primitive into one packed buffer; HBM-write bound."""
@@ -547,6 +593,9 @@ def main():
         return
     if args.workload == "export":
         run_export(args, dev)
+        return
+    if args.workload == "lanes":
+        run_lanes(args, dev)
         return
     if args.copy_probe:
         nb = args.copy_probe << 20

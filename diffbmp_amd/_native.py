@@ -14,7 +14,9 @@ import threading
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdiffbmp_amd.so")
+# DBMP_LIBRARY selects another build of the same sources (e.g. the lane
+# statistics build, libdiffbmp_amd_lanestats.so)
+LIB_PATH = os.environ.get("DBMP_LIBRARY") or os.path.join(_HERE, "libdiffbmp_amd.so")
 
 DBMP_F32, DBMP_F16, DBMP_BF16 = 0, 1, 2
 _DT = {torch.float32: DBMP_F32, torch.float16: DBMP_F16, torch.bfloat16: DBMP_BF16}
@@ -169,6 +171,7 @@ EXPORTS = {
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p]),
     "dbmp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Adam), ctypes.c_void_p]),
+    "dbmp_lane_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32]),
     "dbmp_sincos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                    ctypes.c_void_p]),
     "dbmp_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

@@ -29,18 +29,27 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    """Compile the HIP sources into ``diffbmp_amd/libdiffbmp_amd.so``."""
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SRCS]
+OUT_LANE_STATS = os.path.join(HERE, "libdiffbmp_amd_lanestats.so")
+
+
+def build(verbose: bool = False, lane_stats: bool = False) -> str:
+    """Compile the HIP sources into ``diffbmp_amd/libdiffbmp_amd.so``, or with
+    ``lane_stats`` the profiling build ``libdiffbmp_amd_lanestats.so`` whose
+    raster kernels count pushed lanes per (wave, list entry) (load it with
+    DBMP_LIBRARY)."""
+    out = OUT_LANE_STATS if lane_stats else OUT
+    tmp = out + ".tmp"
+    extra = ["-DDBMP_LANE_STATS"] if lane_stats else []
+    cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-o", tmp, *SRCS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
     if verbose and r.stderr:
         print(r.stderr)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import sys
+    print(build(verbose=True, lane_stats="--lane-stats" in sys.argv))

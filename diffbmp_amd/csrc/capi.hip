@@ -1004,6 +1004,23 @@ int dbmp_adam_step(dbmp_rasterizer* h, const dbmp_adam* cfg, void* stream) {
   return 0;
 }
 
+int dbmp_lane_stats(uint64_t* out, int32_t reset) {
+#ifdef DBMP_LANE_STATS
+  if (!out) return fail(DBMP_E_ARG, "out is NULL");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_lane_stats), sizeof(uint64_t) * 8));
+  if (reset) {
+    const unsigned long long zero[8] = {};
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_lane_stats), zero, sizeof(zero)));
+  }
+  return 0;
+#else
+  (void)out;
+  (void)reset;
+  return fail(DBMP_E_ARG, "this library was built without lane statistics (DBMP_LANE_STATS)");
+#endif
+}
+
 int dbmp_sincos(const float* x, float* s, float* c, int64_t n, void* stream) {
   if (n < 0) return fail(DBMP_E_ARG, "n < 0");
   if (n == 0) return 0;

@@ -147,6 +147,36 @@ __device__ __forceinline__ void blend_color(const RasterArgs& A, const PrimRec& 
   }
 }
 
+// Profiling build (DBMP_LANE_STATS): how many lanes a (wave, list entry)
+// evaluation pushes.  Each wave tallies its evaluations in scalar registers
+// and adds them to these counters once per block: [0] forward evaluations,
+// [1] of them with a pushed lane, [2] pushed lanes; [3..5] the same for the
+// backward.  Off in the library proper (the tallies compile to nothing).
+#ifdef DBMP_LANE_STATS
+static __device__ unsigned long long g_lane_stats[8];
+struct LaneTally {
+  unsigned long long ev = 0, hit = 0, lanes = 0;
+  __device__ __forceinline__ void add(bool pushed) {
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(pushed);
+    ev += 1;
+    hit += m != 0ull ? 1 : 0;
+    lanes += __popcll(m);
+  }
+  __device__ __forceinline__ void flush(int base) const {
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&g_lane_stats[base], ev);
+      atomicAdd(&g_lane_stats[base + 1], hit);
+      atomicAdd(&g_lane_stats[base + 2], lanes);
+    }
+  }
+};
+#else
+struct LaneTally {
+  __device__ __forceinline__ void add(bool) {}
+  __device__ __forceinline__ void flush(int) const {}
+};
+#endif
+
 constexpr int kBatch = 64;    // list entries staged per batch
 constexpr int kThreads = 256; // 4 waves per 16x16 block
 
@@ -377,6 +407,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   float thr = alive ? A.eps : -1.f;   // stop test threshold (below)
   int last = -1;
   int walked = 0;   // entries this wave evaluated (wave-uniform)
+  LaneTally tally;
   int batches = 0;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
@@ -405,6 +436,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         ++walked;
         const PrimRec p = srec[buf][j];
         const Sample<DT> S = eval_sample<DT, false, BLEND>(A, p, fpx, fpy, alive);
+        tally.add(S.pushed);
         if constexpr (BLEND) {
           // skip the colour-map gathers when nothing of this entry reaches the wave
           if (!wave_any(S.pushed)) continue;
@@ -461,6 +493,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     const int4 fl = *reinterpret_cast<const int4*>(salive[buf]);
     if ((fl.x | fl.y | fl.z | fl.w) == 0) break;
   }
+  tally.flush(0);
   if (G.px < A.W && G.py < A.H) {
     const long p = (long)G.py * A.W + G.px;
     out_color[3 * p + 0] = nz ? Cf0 : C0;
@@ -573,6 +606,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G);
   __syncthreads();
   const int stop = start + min(skmax, end - start);
+  LaneTally tally;
   int buf = 0;
   for (int base = start; base < stop; base += kBatch, buf ^= 1) {
     const bool have_next = base + kBatch < stop;
@@ -592,6 +626,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const int kk = base - start + j;
       const PrimRec p = srec[buf][j];
       const Sample<DT> S = eval_sample<DT, true, BLEND>(A, p, fpx, fpy, kk < lst);
+      tally.add(S.pushed);
       if (!wave_any(S.pushed)) continue;
       float col[3], dcx[3] = {0.f, 0.f, 0.f}, dcy[3] = {0.f, 0.f, 0.f};
       const bool has_map = BLEND && A.cmap != nullptr;
@@ -694,6 +729,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     }
     __syncthreads();
   }
+  tally.flush(3);
   if constexpr (LOSS) {
     // one loss atomic per workgroup, issued when its list walk is done (the
     // workgroups finish at different times, so the same-address atomics do

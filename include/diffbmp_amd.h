@@ -422,6 +422,14 @@ int dbmp_psd_layers(const dbmp_layer_prims* prims, const float* templates, int n
  * values (the oracle compiles the same source, csrc/dbmp_trig.h). */
 int dbmp_sincos(const float* x, float* s, float* c, int64_t n, void* stream);
 
+/* Profiling counters of a library built with DBMP_LANE_STATS (python -m
+ * diffbmp_amd.build --lane-stats): out[0..2] forward (wave, list entry)
+ * evaluations, those with at least one pushed lane, pushed lanes; out[3..5]
+ * the same for the backward kernels of dbmp_forward / dbmp_backward*.
+ * Synchronises the device; reset zeroes the counters.  The library proper
+ * returns DBMP_E_ARG.  No counterpart in the reference. */
+int dbmp_lane_stats(uint64_t* out, int32_t reset);
+
 #ifdef __cplusplus
 }
 #endif
