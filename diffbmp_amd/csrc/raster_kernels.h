@@ -823,25 +823,36 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd_cands(
                                      Tfin + px * b, lastin + px * b, grad8 + 8L * A.N * b);
 }
 
-// Heaviest-first order of nb blocks from their forward costs: 64 cost
-// buckets (bucket 0 the heaviest), blocks within a bucket in any order.
-// One 1024-thread workgroup (16 waves: the kernel is latency bound, so
-// more waves mean fewer dependent rounds).  The costs are staged in LDS with every load in
-// flight; lanes of a wave that fall in the same bucket are found with ballots
-// (one per bucket bit) and counted with a single LDS atomic, so a crowded
-// bucket costs one atomic per wave, not one per lane.  Launching the heavy
-// blocks first keeps the last workgroups of a launch short, so the machine
-// stays full until the end.  (Ordering whole super-tile groups instead was
-// measured: the order kernel halves, the raster kernels lose more.)
+// Launch order of nb blocks from their forward costs, heaviest first.
+// Consecutive workgroups are dealt round-robin to the 8 XCDs (workgroup w to
+// XCD w mod 8).  With DBMP_ORDER_BANDED (and nb a multiple of 8) the blocks
+// are cut into 8 contiguous bands of natural (row-major) order and band x's
+// i-th heaviest block is launched at position 8 i + x: each XCD then works
+// through its own band of neighbouring tiles -- sharing its L2 over their
+// primitives and texels -- heaviest first.  Otherwise the order is global
+// heaviest first: 64 cost buckets (bucket 0 the heaviest), blocks within a
+// bucket in any order.  One 1024-thread workgroup (16 waves: the kernel is
+// latency bound, so more waves mean fewer dependent rounds).  The costs are
+// staged in LDS with every load in flight; lanes of a wave with the same key
+// are found with ballots (one per key bit) and counted with a single LDS
+// atomic, so a crowded bucket costs one atomic per wave, not one per lane.
+#ifndef DBMP_ORDER_BANDED
+#define DBMP_ORDER_BANDED 0
+#endif
 constexpr int kOrderThreads = 1024;
-constexpr int kOrderBits = 6;
+constexpr int kOrderBits = 6;                       // cost buckets
+constexpr int kOrderKeyBits = kOrderBits + 3;       // + the band (8 XCDs)
+constexpr int kOrderKeys = 1 << kOrderKeyBits;
 constexpr int kOrderLds = 16384;   // costs staged in LDS up to this many blocks
 static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int* __restrict__ cost_g,
                                                                     int nb, int* __restrict__ order) {
-  __shared__ int cnt[1 << kOrderBits];
-  __shared__ int wmax[kOrderThreads / 64];
+  __shared__ int cnt[kOrderKeys];
+  __shared__ int wsum[kOrderThreads / 64];
   __shared__ int scost[kOrderLds];
-  if (threadIdx.x < (1 << kOrderBits)) cnt[threadIdx.x] = 0;
+  const bool banded = DBMP_ORDER_BANDED && (nb & 7) == 0;
+  const int nkeys = banded ? kOrderKeys : (1 << kOrderBits);
+  const int kbits = banded ? kOrderKeyBits : kOrderBits;
+  for (int i = threadIdx.x; i < kOrderKeys; i += kOrderThreads) cnt[i] = 0;
   const bool staged = nb <= kOrderLds;
   const int* cost = staged ? scost : cost_g;
   int m = 1;
@@ -862,61 +873,76 @@ static __global__ __launch_bounds__(kOrderThreads) void k_block_order(const int*
     for (int i = threadIdx.x; i < nb; i += kOrderThreads) m = max(m, cost_g[i]);
   }
   m = wave_max_i32(m);
-  if (lane_id() == 0) wmax[threadIdx.x >> 6] = m;
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = m;
   __syncthreads();
-  int mx = wmax[0];
+  int mx = wsum[0];
 #pragma unroll
-  for (int w = 1; w < kOrderThreads / 64; ++w) mx = max(mx, wmax[w]);
+  for (int w = 1; w < kOrderThreads / 64; ++w) mx = max(mx, wsum[w]);
   constexpr int kTop = (1 << kOrderBits) - 1;
   const float qs = (float)kTop / (float)mx;
-  auto bucket = [&](int c) {   // 0 for the heaviest costs; any monotone map serves
-    return kTop - min((int)((float)max(c, 0) * qs), kTop);
+  const int band_len = nb >> 3;
+  // key: (band, bucket) when banded, else the bucket (0 for the heaviest)
+  auto key_of = [&](int i) {
+    const int bucket = kTop - min((int)((float)max(cost[i], 0) * qs), kTop);
+    return banded ? ((i / band_len) << kOrderBits) | bucket : bucket;
   };
   const int l = lane_id();
   const unsigned long long lt = (1ull << l) - 1ull;
-  // lanes of this wave with the same bucket as this lane (among valid lanes)
-  auto peers_of = [&](bool ok, int b) {
+  // lanes of this wave with the same key as this lane (among valid lanes)
+  auto peers_of = [&](bool ok, int key) {
     unsigned long long p = __builtin_amdgcn_ballot_w64(ok);
-#pragma unroll
-    for (int k = 0; k < kOrderBits; ++k) {
-      const bool bit = (b >> k) & 1;
+    for (int k = 0; k < kbits; ++k) {
+      const bool bit = (key >> k) & 1;
       const unsigned long long mk = __builtin_amdgcn_ballot_w64(bit);
       p &= bit ? mk : ~mk;
     }
     return p;
   };
+  __syncthreads();   // wsum is reused by the scan below
   const int rounds = (nb + kOrderThreads - 1) / kOrderThreads;
 #pragma unroll 4
   for (int r = 0; r < rounds; ++r) {
     const int i = r * kOrderThreads + threadIdx.x;
     const bool ok = i < nb;
-    const int b = ok ? bucket(cost[i]) : 0;
-    const unsigned long long p = peers_of(ok, b);
-    if (ok && (p & lt) == 0ull) atomicAdd(&cnt[b], __popcll(p));
+    const int key = ok ? key_of(i) : 0;
+    const unsigned long long p = peers_of(ok, key);
+    if (ok && (p & lt) == 0ull) atomicAdd(&cnt[key], __popcll(p));
   }
   __syncthreads();
-  if (threadIdx.x < 64) {   // exclusive scan of the 64 bucket counts in wave 0
-    const int c = cnt[threadIdx.x];
-    int inc = c;
+  // exclusive scan of the key counts, one per thread of the first waves
+  static_assert(kOrderKeys <= kOrderThreads, "one key per thread");
+  const int t = threadIdx.x;
+  const int c = t < nkeys ? cnt[t] : 0;
+  int inc = c;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(inc, o, 64);
-      if (l >= o) inc += t;
-    }
-    cnt[threadIdx.x] = inc - c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (l >= o) inc += u;
   }
+  if (l == 63) wsum[t >> 6] = inc;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < (t >> 6); ++w) before += wsum[w];
+  if (t < nkeys) cnt[t] = before + inc - c;
   __syncthreads();
 #pragma unroll 4
   for (int r = 0; r < rounds; ++r) {
     const int i = r * kOrderThreads + threadIdx.x;
     const bool ok = i < nb;
-    const int b = ok ? bucket(cost[i]) : 0;
-    const unsigned long long p = peers_of(ok, b);
+    const int key = ok ? key_of(i) : 0;
+    const unsigned long long p = peers_of(ok, key);
     const int leader = ok ? __builtin_ctzll(p) : 0;
     int base = 0;
-    if (ok && l == leader) base = atomicAdd(&cnt[b], __popcll(p));
+    if (ok && l == leader) base = atomicAdd(&cnt[key], __popcll(p));
     base = __shfl(base, leader, 64);
-    if (ok) order[base + __popcll(p & lt)] = i;
+    if (ok) {
+      int pos = base + __popcll(p & lt);
+      if (banded) {   // rank in its band -> launch position 8 rank + band
+        const int band = key >> kOrderBits;
+        pos = 8 * (pos - band * band_len) + band;
+      }
+      order[pos] = i;
+    }
   }
 }
 
