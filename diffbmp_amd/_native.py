@@ -18,8 +18,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # statistics build, libdiffbmp_amd_lanestats.so)
 LIB_PATH = os.environ.get("DBMP_LIBRARY") or os.path.join(_HERE, "libdiffbmp_amd.so")
 
-DBMP_F32, DBMP_F16, DBMP_BF16 = 0, 1, 2
+DBMP_F32, DBMP_F16, DBMP_BF16, DBMP_F16_APP, DBMP_BF16_APP = 0, 1, 2, 3, 4
 _DT = {torch.float32: DBMP_F32, torch.float16: DBMP_F16, torch.bfloat16: DBMP_BF16}
+# fp32 geometry with half-precision opacities and colours
+_DT_APP = {torch.float16: DBMP_F16_APP, torch.bfloat16: DBMP_BF16_APP}
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -306,10 +308,18 @@ class Handle:
     @staticmethod
     def prims(means2D, radii, rotations, opacities, colors, bmp_sel, reuse: bool = False) -> Prims:
         dt = _DT[means2D.dtype]
-        for t, nm in ((radii, "radii"), (rotations, "rotations"), (opacities, "opacities"),
-                      (colors, "colors")):
+        for t, nm in ((radii, "radii"), (rotations, "rotations")):
             if t.dtype != means2D.dtype:
                 raise TypeError(f"{nm} dtype {t.dtype} != means2D dtype {means2D.dtype}")
+        if colors.dtype != opacities.dtype:
+            raise TypeError(f"colors dtype {colors.dtype} != opacities dtype {opacities.dtype}")
+        if opacities.dtype != means2D.dtype:
+            # fp32 positions, radii and rotations with fp16 / bf16 opacities and colours
+            if means2D.dtype != torch.float32 or opacities.dtype not in _DT_APP:
+                raise TypeError(f"opacities dtype {opacities.dtype} with means2D dtype "
+                                f"{means2D.dtype}: mixed storage is fp32 geometry with fp16 or bf16 "
+                                "appearance")
+            dt = _DT_APP[opacities.dtype]
         N = radii.shape[0]
         if means2D.shape != (N, 2) or colors.shape != (N, 3) or rotations.shape != (N,) \
                 or opacities.shape != (N,) or bmp_sel.shape != (N,):

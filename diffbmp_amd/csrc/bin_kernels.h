@@ -175,11 +175,13 @@ __device__ __forceinline__ void prep_core(float x, float y, float r, float th, f
   }
 }
 
-template <typename PT>
+// GT: storage type of positions, radii and rotations; AT: of opacities and
+// colours (the same, or fp32 geometry with half-precision appearance).
+template <typename GT, typename AT = GT>
 static __global__ __launch_bounds__(256) void k_prep(
-    int N, const PT* __restrict__ means2D, const PT* __restrict__ radii,
-    const PT* __restrict__ rotations, const PT* __restrict__ opacities,
-    const PT* __restrict__ colors, const int* __restrict__ sel, const float* __restrict__ ext,
+    int N, const GT* __restrict__ means2D, const GT* __restrict__ radii,
+    const GT* __restrict__ rotations, const AT* __restrict__ opacities,
+    const AT* __restrict__ colors, const int* __restrict__ sel, const float* __restrict__ ext,
     int T, int Ht, int Wt, int H, int W, int ts, int tilesX, int tilesY, float alpha_ub,
     int bin_shift, PrimRec* __restrict__ recs, int* __restrict__ counts,
     int4* __restrict__ ranges) {

@@ -188,7 +188,8 @@ const int* block_order(const dbmp_rasterizer* h, int nb) {
 int check_prims(const dbmp_prims* p) {
   if (!p) return fail(DBMP_E_ARG, "prims is NULL");
   if (p->num_primitives < 0) return fail(DBMP_E_ARG, "num_primitives < 0");
-  if (p->dtype < DBMP_F32 || p->dtype > DBMP_BF16) return fail(DBMP_E_ARG, "bad prims dtype %d", p->dtype);
+  if (p->dtype < DBMP_F32 || p->dtype > DBMP_BF16_APP)
+    return fail(DBMP_E_ARG, "bad prims dtype %d", p->dtype);
   if (p->num_primitives > 0 && (!p->means2D || !p->radii || !p->rotations || !p->opacities ||
                                 !p->colors || !p->bmp_sel))
     return fail(DBMP_E_ARG, "a primitive parameter pointer is NULL");
@@ -221,17 +222,30 @@ int launch_prep(dbmp_rasterizer* h, const dbmp_prims* p, const float* ext, int T
   h->recs_n = recs_at == nullptr ? N : -1;
   if (N == 0) return 0;
   const dim3 grid(ceil_div(N, 256)), block(256);
-#define PREP_ARGS(TY)                                                                          \
-  N, (const TY*)p->means2D, (const TY*)p->radii, (const TY*)p->rotations,                      \
-      (const TY*)p->opacities, (const TY*)p->colors, p->bmp_sel, ext, T, Ht, Wt, h->H, h->W,   \
+#define PREP_ARGS(GT, AT)                                                                      \
+  N, (const GT*)p->means2D, (const GT*)p->radii, (const GT*)p->rotations,                      \
+      (const AT*)p->opacities, (const AT*)p->colors, p->bmp_sel, ext, T, Ht, Wt, h->H, h->W,   \
       h->ts, h->tilesX, h->tilesY, h->alpha_ub, shift, recs_at ? recs_at : h->recs, h->counts, \
       ranges ? ranges : h->ranges
-  if (p->dtype == DBMP_F32)
-    hipLaunchKernelGGL(k_prep<float>, grid, block, 0, st, PREP_ARGS(float));
-  else if (p->dtype == DBMP_F16)
-    hipLaunchKernelGGL(k_prep<__half>, grid, block, 0, st, PREP_ARGS(__half));
-  else
-    hipLaunchKernelGGL(k_prep<__hip_bfloat16>, grid, block, 0, st, PREP_ARGS(__hip_bfloat16));
+  switch (p->dtype) {
+    case DBMP_F32:
+      hipLaunchKernelGGL((k_prep<float, float>), grid, block, 0, st, PREP_ARGS(float, float));
+      break;
+    case DBMP_F16:
+      hipLaunchKernelGGL((k_prep<__half, __half>), grid, block, 0, st, PREP_ARGS(__half, __half));
+      break;
+    case DBMP_BF16:
+      hipLaunchKernelGGL((k_prep<__hip_bfloat16, __hip_bfloat16>), grid, block, 0, st,
+                         PREP_ARGS(__hip_bfloat16, __hip_bfloat16));
+      break;
+    case DBMP_F16_APP:
+      hipLaunchKernelGGL((k_prep<float, __half>), grid, block, 0, st, PREP_ARGS(float, __half));
+      break;
+    default:
+      hipLaunchKernelGGL((k_prep<float, __hip_bfloat16>), grid, block, 0, st,
+                         PREP_ARGS(float, __hip_bfloat16));
+      break;
+  }
 #undef PREP_ARGS
   LAUNCH_CHECK("k_prep");
   return 0;

@@ -36,7 +36,8 @@ class FitJob:
     Adam master weights stay fp32.  ``bg`` is a (H,W,3) tensor or None for white.
     ``max_steps`` sizes the on-device loss history.  ``exact_color_map_grad``
     selects the exact gradient of the colour-map blend (``c_blend > 0``)
-    instead of the reference's.
+    instead of the reference's.  ``fp32_geometry`` keeps positions, radii and
+    rotations in fp32 when ``store_dtype`` is fp16 / bf16 (mixed storage).
     """
 
     def __init__(self, means2D, radii, rotations, opacities, colors, templates, bmp_sel, target,
@@ -44,7 +45,7 @@ class FitJob:
                  eps_T: float = 1e-6, tile_size: int = 16, alpha_upper_bound: float = 1.0,
                  clamp: bool = True, bg=None, max_steps: int = 1000, handle: Handle | None = None,
                  colors_orig=None, c_blend: float = 0.0, colors_orig_per_template: bool = True,
-                 exact_color_map_grad: bool = False):
+                 exact_color_map_grad: bool = False, fp32_geometry: bool = False):
         dev = means2D.device
         _native.require_gpu(means2D, "means2D")
         H, W = int(target.shape[0]), int(target.shape[1])
@@ -63,10 +64,16 @@ class FitJob:
         # gradient rows (N,8): x, y, r, theta, opacity logit, 3 colour logits.
         # The backward adds into them and the Adam step zeroes what it reads.
         self.grad_rows = torch.zeros((self.N, 8), dtype=f32, device=dev)
+        # fp32_geometry: positions, radii and rotations keep their fp32 master
+        # values (the rasterizer reads them directly); only opacities, colours
+        # and the bitmaps are stored in store_dtype
+        geom = ("means2D", "radii", "rotations")
+        self.fp32_geometry = bool(fp32_geometry) and store_dtype != f32
         if store_dtype == f32:
             self.store = self.params
         else:
-            self.store = {k: v.to(store_dtype) for k, v in self.params.items()}
+            self.store = {k: (v if self.fp32_geometry and k in geom else v.to(store_dtype))
+                          for k, v in self.params.items()}
         self.sel = bmp_sel.to(dev, torch.int32).contiguous()
         self.templates, self.ext, self.tmpl_version = self.handle.prepared_templates(
             templates.to(dev), store_dtype)
@@ -96,7 +103,8 @@ class FitJob:
             col, comps = _ROW_COLUMN[k]
             g.grad = self.grad_rows.data_ptr() + 4 * col
             g.grad_row, g.grad_comps, g.zero_grad = 8, comps, 1
-            g.store = self.store[k].data_ptr() if store_dtype != f32 else None
+            g.store = (self.store[k].data_ptr()
+                       if store_dtype != f32 and not (self.fp32_geometry and k in geom) else None)
             g.numel = p.numel()
             g.lr = float(lrs[k])
             g.stride = 2 if k == "means2D" else 1

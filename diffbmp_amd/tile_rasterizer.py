@@ -118,7 +118,11 @@ class _Run:
         for t, nm in ((means2D, "means2D"), (templates, "primitive_templates")):
             _native.require_gpu(t, nm)
         cv = lambda t: t.to(dtype=store_dtype).contiguous()
-        self.means2D, self.radii, self.rotations = cv(means2D), cv(radii), cv(rotations)
+        # mixed storage: fp32 positions, radii and rotations, half-precision
+        # opacities, colours and bitmaps
+        cg = (lambda t: t.to(dtype=torch.float32).contiguous()) \
+            if getattr(handle, "fp32_geometry", False) else cv
+        self.means2D, self.radii, self.rotations = cg(means2D), cg(radii), cg(rotations)
         self.opacities, self.colors = cv(opacities), cv(colors)
         self.templates, self.ext, self.tmpl_version = handle.prepared_templates(templates, store_dtype)
         self.sel = bmp_sel.to(torch.int32).contiguous()
@@ -335,10 +339,15 @@ class TileRasterizer:
     """Class-based rasterizer (``cuda_tile_rasterizer/__init__.py:60-121``)."""
 
     def __init__(self, image_height, image_width, tile_size=16, sigma=0.0, alpha_upper_bound=1.0,
-                 max_prims_per_pixel=151, num_primitives=None, use_fp16=False, storage_dtype=None):
+                 max_prims_per_pixel=151, num_primitives=None, use_fp16=False, storage_dtype=None,
+                 fp32_geometry=False):
         """``storage_dtype`` (extension): torch.float16 or torch.bfloat16 selects the
         reduced-precision storage of the ``use_fp16`` path; torch.float32 with
-        ``use_fp16=False`` is the plain path."""
+        ``use_fp16=False`` is the plain path.  ``fp32_geometry`` (extension): in
+        the reduced-precision path keep positions, radii and rotations in fp32
+        (the reference's fp16 path casts means2D to half,
+        ``cuda_tile_rasterizer/__init__.py:348``, which moves a 1024-px
+        coordinate by up to 0.25 px)."""
         if storage_dtype is not None and storage_dtype != torch.float32:
             if storage_dtype not in (torch.float16, torch.bfloat16):
                 raise ValueError(f"unsupported storage_dtype {storage_dtype}")
@@ -356,6 +365,7 @@ class TileRasterizer:
                              max_prims_per_pixel, num_primitives)
         self.handle.eps_T = DEFAULT_EPS_T
         self.handle.blend_grad_exact = False
+        self.handle.fp32_geometry = bool(fp32_geometry) and use_fp16
         self.handle.store_dtype = (storage_dtype if storage_dtype is not None and use_fp16
                                    else (torch.float16 if use_fp16 else torch.float32))
         with _ACTIVE_LOCK:

@@ -33,6 +33,11 @@ extern "C" {
 #define DBMP_F32 0
 #define DBMP_F16 1
 #define DBMP_BF16 2
+/* dbmp_prims only: positions, radii and rotations fp32; opacities and colours
+ * fp16 / bf16 (half-precision positions move a 1024-px coordinate by up to
+ * 0.25 px / 2 px; the appearance values do not need that range) */
+#define DBMP_F16_APP 3
+#define DBMP_BF16_APP 4
 
 #define DBMP_OK 0
 #define DBMP_E_ARG 1      /* invalid argument / unsupported shape */
@@ -41,7 +46,8 @@ extern "C" {
 
 typedef struct dbmp_rasterizer dbmp_rasterizer;
 
-/* Per-primitive parameters, storage dtype DBMP_F32 / F16 / BF16.
+/* Per-primitive parameters, storage dtype DBMP_F32 / F16 / BF16, or
+ * DBMP_F16_APP / DBMP_BF16_APP (fp32 geometry, half-precision appearance).
  * reuse_records: nonzero asserts that these exact parameter values were the
  * input of the previous dbmp_bin / dbmp_forward / dbmp_backward call on this
  * handle, so the per-primitive constants computed there are reused. */

@@ -9,19 +9,22 @@ from diffbmp_amd import _native
 from diffbmp_amd._native import Handle
 
 
-def to_dev(scene, dev, dtype=torch.float32):
+def to_dev(scene, dev, dtype=torch.float32, fp32_geometry=False):
     t = lambda a, dt=dtype: torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(dt)
-    return dict(means2D=t(scene.means2D), radii=t(scene.radii), rotations=t(scene.rotations),
+    g = torch.float32 if fp32_geometry else dtype
+    return dict(means2D=t(scene.means2D, g), radii=t(scene.radii, g), rotations=t(scene.rotations, g),
                 opacities=t(scene.opacities), colors=t(scene.colors),
                 templates=t(scene.templates), bmp_sel=t(scene.bmp_sel, torch.int32))
 
 
-def rounded(scene, dtype):
-    """Scene whose float arrays are rounded to `dtype` storage and back to fp32."""
+def rounded(scene, dtype, fp32_geometry=False):
+    """Scene whose float arrays are rounded to `dtype` storage and back to fp32
+    (positions, radii and rotations left in fp32 with ``fp32_geometry``)."""
     import copy
     s = copy.copy(scene)
     r = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dtype).float().numpy()
-    s.means2D, s.radii, s.rotations = r(scene.means2D), r(scene.radii), r(scene.rotations)
+    if not fp32_geometry:
+        s.means2D, s.radii, s.rotations = r(scene.means2D), r(scene.radii), r(scene.rotations)
     s.opacities, s.colors, s.templates = r(scene.opacities), r(scene.colors), r(scene.templates)
     if scene.colors_orig_tmpl is not None:
         s.colors_orig_tmpl = r(scene.colors_orig_tmpl)
@@ -32,13 +35,14 @@ class GpuRun:
     """Bins, renders and back-propagates one scene through the C ABI."""
 
     def __init__(self, scene, dev, dtype=torch.float32, eps_T=0.0, ts=16, co_mode=0,
-                 bm_dtype=None, exact_blend=False):
-        """``dtype``: parameter storage; ``bm_dtype``: bitmap and colour-map
-        storage (default ``dtype``)."""
+                 bm_dtype=None, exact_blend=False, fp32_geometry=False):
+        """``dtype``: parameter storage (positions, radii and rotations fp32
+        with ``fp32_geometry``); ``bm_dtype``: bitmap and colour-map storage
+        (default ``dtype``)."""
         self.scene, self.dev, self.dtype, self.eps = scene, dev, dtype, eps_T
         bm_dtype = dtype if bm_dtype is None else bm_dtype
         self.h = Handle(scene.H, scene.W, ts, 0.0, scene.alpha_ub, 151, scene.N)
-        self.d = to_dev(scene, dev, dtype)
+        self.d = to_dev(scene, dev, dtype, fp32_geometry)
         d = self.d
         d["templates"] = d["templates"].to(bm_dtype)
         self.prims = Handle.prims(d["means2D"], d["radii"], d["rotations"], d["opacities"],

@@ -165,6 +165,35 @@ def test_fitjob_reduced_precision_runs(dev, store):
         assert torch.equal(job.store[k], job.params[k].to(store))
 
 
+@pytest.mark.parametrize("store", [torch.float16, torch.bfloat16])
+def test_fitjob_mixed_storage_tracks_fp32(dev, store):
+    """Mixed storage (fp32 positions, radii, rotations; half-precision
+    opacities, colours, bitmaps): the half copies follow the fp32 masters, the
+    geometry is the master itself, and the first image is within the storage
+    rounding of the fp32 job's."""
+    sc = S.make_scene(0, with_target=False)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    target = torch.full((sc.H, sc.W, 3), 0.5, device=dev)
+    init = [t(a) for a in (sc.means2D, sc.radii, sc.rotations, sc.opacities, sc.colors)]
+    mixed = FitJob(*init, t(sc.templates), t(sc.bmp_sel), target, lr=1e-2, store_dtype=store,
+                   fp32_geometry=True)
+    ref = FitJob(*init, t(sc.templates), t(sc.bmp_sel), target, lr=1e-2)
+    mixed.step()
+    ref.step()
+    torch.cuda.synchronize()
+    e = (mixed.color - ref.color).abs().max().item()
+    print(f"{store}: first image vs fp32 {e:.2e}")
+    assert e <= (2e-3 if store == torch.float16 else 1.6e-2)
+    for _ in range(9):
+        mixed.step()
+    for k in ("means2D", "radii", "rotations"):
+        assert mixed.store[k] is mixed.params[k]
+    for k in ("opacities", "colors"):
+        assert torch.equal(mixed.store[k], mixed.params[k].to(store))
+    l = mixed.loss_history().cpu().numpy()
+    assert np.all(np.isfinite(l)) and l[-1] < l[0]
+
+
 def c0_target(dev):
     """The plain workload's target: the render over white of a second random
     256-primitive parameter set (role-1 seed) through the drop-in rasterizer."""

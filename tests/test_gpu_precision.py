@@ -6,7 +6,9 @@ For the 1024^2 / 8192-primitive workload, on a random sample of tiles:
   * total error: HIP vs the oracle on the original fp32 inputs -- adds the
     storage quantisation (fp16 positions move in 0.5 px steps at 512-1024 px,
     bf16 in 4 px steps; the reference's fp16 path casts means2D to half too,
-    cuda_tile_rasterizer/__init__.py:348).  Reported, bounded loosely.
+    cuda_tile_rasterizer/__init__.py:348).  Reported, bounded loosely for
+    half-precision positions; with mixed storage (fp32 positions, radii and
+    rotations) the fp16 image error is bounded by the FP16 bar 2e-3.
 The error table is printed, and also merged into the JSON file named by the
 environment variable DBMP_PRECISION_TABLE when it is set.
 """
@@ -36,11 +38,14 @@ def _errs(img, ref_img, g, ref_g, pm):
     return out
 
 
-@pytest.mark.parametrize("store", ["fp16", "bf16", "fp32"])
+@pytest.mark.parametrize("store", ["fp16", "bf16", "fp32", "fp16-app", "bf16-app"])
 def test_precision_error_table(dev, store):
-    dt = {"fp16": torch.float16, "bf16": torch.bfloat16, "fp32": torch.float32}[store]
+    """``-app``: mixed storage, positions / radii / rotations in fp32 and only
+    opacities, colours and bitmaps in half precision."""
+    dt = {"fp16": torch.float16, "bf16": torch.bfloat16, "fp32": torch.float32}[store.split("-")[0]]
+    mixed = store.endswith("-app")
     sc = S.make_scene(2, with_target=False)
-    run = GpuRun(sc, dev, dtype=dt, eps_T=1e-6)
+    run = GpuRun(sc, dev, dtype=dt, eps_T=1e-6, fp32_geometry=mixed)
     col, alp = run.forward()
     rng = np.random.default_rng(17)
     tiles = np.sort(rng.choice(64 * 64, 16, replace=False)).astype(np.int32)
@@ -54,7 +59,7 @@ def test_precision_error_table(dev, store):
     img = np.concatenate([col, alp[..., None]], -1)
     table = {}
     offs, idx = run.offs.cpu().numpy(), run.idx.cpu().numpy()
-    for name, scn in (("arithmetic", rounded(sc, dt)), ("total", sc)):
+    for name, scn in (("arithmetic", rounded(sc, dt, mixed)), ("total", sc)):
         osc = O.scene_from(scn, offs, idx, tile_subset=tiles)
         rc, ra, _ = O.forward(osc)
         ref_g = O.backward(osc, gC.astype(np.float64), gA.astype(np.float64))
@@ -69,5 +74,12 @@ def test_precision_error_table(dev, store):
     a = table["arithmetic"]
     assert a["image_max_abs"] <= 1e-5
     assert max(a[f"grad_{k}_normwise"] for k in GRADS) <= 1e-4
+    t = table["total"]
     if store == "fp16":
-        assert table["total"]["image_mean_abs"] <= 2e-3
+        assert t["image_mean_abs"] <= 2e-3
+    if store == "fp16-app":
+        # with fp32 geometry the storage error of the image is the fp16 rounding
+        # of opacity / colour logits and bitmap texels: within the FP16 bar
+        assert t["image_max_abs"] <= 2e-3
+    if store == "bf16-app":
+        assert t["image_max_abs"] <= 1.6e-2
