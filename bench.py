@@ -62,6 +62,8 @@ def parse():
                     help="scale: the headline fit step; stress: the 2048^2 / 32768-primitive "
                          "RGBA colour-mapped fit step; serving: a Poisson stream of fit jobs "
                          "through the batched multi-job engine (each its own JSON line)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the fp32 / fp16 / mixed-storage rate and PSNR comparison")
     ap.add_argument("--no-serving", action="store_true",
                     help="leave the bounded serving run out of the headline line")
     ap.add_argument("--serve-jobs", type=int, default=48)
@@ -234,8 +236,10 @@ def run_fused(args, sc, dev, sdt):
     return job, job.step, job.handle, fwd_bwd, reset
 
 
-def run_autograd(args, sc, dev, sdt):
-    """The reference-compatible TileRasterizer behind torch autograd."""
+def run_autograd(args, sc, dev, sdt, fused_loss=False):
+    """The reference-compatible TileRasterizer behind torch autograd, with the
+    background composite + MSE in torch as the reference's renderer writes it,
+    or (`fused_loss`) through the one-kernel composite_mse_loss op."""
     import torch
     import torch.nn.functional as F
     from diffbmp_amd.tile_rasterizer import TileRasterizer
@@ -255,7 +259,10 @@ def run_autograd(args, sc, dev, sdt):
     def fwd_bwd():
         opt.zero_grad(set_to_none=True)
         color, alpha = rast(*leaves, empty, templates, sel, 0.0, lr_conf, None)
-        loss = F.mse_loss(color.float() + (1.0 - alpha.float()).unsqueeze(-1) * bg, target)
+        if fused_loss:
+            loss = rast.composite_mse_loss(color, alpha, target)
+        else:
+            loss = F.mse_loss(color.float() + (1.0 - alpha.float()).unsqueeze(-1) * bg, target)
         loss.backward()
 
     def step():
@@ -574,6 +581,45 @@ def run_export(args, dev):
     print(json.dumps(out))
 
 
+def precision_summary(sc, dev, steps=200, warmup=20, psnr_steps=200):
+    """Step rate of the fused step with fp32, fp16 and mixed storage (fp32
+    positions, radii and rotations; fp16 opacities, colours and bitmaps), each
+    as a ratio to fp32, and the PSNR (dB) each reaches after `psnr_steps` Adam
+    steps from the same start."""
+    import torch
+    from diffbmp_amd.fit import FitJob
+    kinds = (("fp32", torch.float32, False), ("fp16", torch.float16, False),
+             ("fp16-app", torch.float16, True))
+    out = {}
+    for name, dt, geom32 in kinds:
+        pr = make_problem(sc, dev, dt)
+        job = FitJob(*pr["params"], pr["templates"], pr["sel"], pr["target"], lr=1e-2, store_dtype=dt,
+                     fp32_geometry=geom32, max_steps=warmup + steps + 1)
+        for _ in range(warmup):
+            job.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            job.step()
+        torch.cuda.synchronize()
+        rate = steps / (time.perf_counter() - t0)
+        del job
+        pr = make_problem(sc, dev, dt)
+        job = FitJob(*pr["params"], pr["templates"], pr["sel"], pr["target"], lr=1e-2, store_dtype=dt,
+                     fp32_geometry=geom32, max_steps=psnr_steps + 1)
+        for _ in range(psnr_steps):
+            job.step()
+        img = job.render()
+        mse = float(((img - pr["target"]) ** 2).mean().item())
+        out[name] = {"steps_per_s": round(rate, 2), "psnr_db": round(FitJob.psnr(mse), 4)}
+        del job
+    for name in out:
+        out[name]["speed_vs_fp32"] = round(out[name]["steps_per_s"] / out["fp32"]["steps_per_s"], 4)
+        out[name]["psnr_minus_fp32_db"] = round(out[name]["psnr_db"] - out["fp32"]["psnr_db"], 4)
+    out["psnr_steps"] = psnr_steps
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -612,6 +658,7 @@ def main():
     res = time_mode(args, runners[args.mode], sc, dev, sdt)
     other = "autograd" if args.mode == "fused" else "fused"
     res_other = time_mode(args, runners[other], sc, dev, sdt)
+    res_fl = time_mode(args, lambda a, s, d, t: run_autograd(a, s, d, t, fused_loss=True), sc, dev, sdt)
     eps_T = args.eps_t if args.eps_t is not None else 1e-6
     P, list_mean, walked = list_stats(res["handle"], sc, dev, sdt, eps_T)
 
@@ -656,6 +703,10 @@ def main():
         # update and clamp)
         "fwd_bwd_iters_per_s": round(1.0 / res["fwd_bwd_s"], 3),
         f"{other}_fwd_bwd_iters_per_s": round(1.0 / res_other["fwd_bwd_s"], 3),
+        # the drop-in autograd path with the composite + MSE in one kernel
+        # (composite_mse_loss) instead of torch's elementwise and reduction ops
+        "autograd_fused_loss_iters_per_s": round(args.steps / res_fl["total"], 3),
+        "autograd_fused_loss_fwd_bwd_iters_per_s": round(1.0 / res_fl["fwd_bwd_s"], 3),
         "samples_per_s_fwd": float(f"{walked * H * W / res['fwd_s']:.4g}"),
         "pairs": P,
         "list_len_mean": round(list_mean, 2),
@@ -682,6 +733,8 @@ def main():
                                      "source": "profiles/pmc_valu_scale_fp32.json"}
     if args.psnr_steps:
         out["psnr"] = psnr_compare(args, sc, dev, sdt)
+    if not args.no_precision:
+        out["storage_precision"] = precision_summary(sc, dev)
     out["plain_fit"] = gpu_plain_fit(dev)
     if not args.no_serving:
         out["serving"] = serving_summary(dev)

@@ -969,9 +969,18 @@ int dbmp_loss_mse(dbmp_rasterizer* h, const float* color, const float* alpha, co
     return fail(DBMP_E_ARG, "NULL image pointer");
   const long long npix = (long long)h->H * h->W;
   const float inv = 1.f / (float)(3 * npix);
-  const int grid = (int)std::min<long long>(ceil_div(npix, 256), 2048);
-  hipLaunchKernelGGL(k_loss_mse, dim3(grid), dim3(256), 0, (hipStream_t)stream, color, alpha, bg,
-                     bg_value, target, npix, inv, grad_color, grad_alpha, loss_accum);
+  const auto a16 = [](const void* q) { return ((uintptr_t)q & 15u) == 0; };
+  const bool vec = a16(color) && a16(alpha) && a16(target) && a16(grad_color) && a16(grad_alpha) &&
+                   (!bg || a16(bg));
+  // kLossGroups float4 groups (4 pixels each) per thread on the vector path
+  const long long work = vec ? ceil_div(std::max(npix >> 2, 1LL), (long long)kLossGroups) : npix;
+  const int grid = (int)std::min<long long>(ceil_div(work, 256), 16384);
+  if (vec)
+    hipLaunchKernelGGL(k_loss_mse<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, color, alpha,
+                       bg, bg_value, target, npix, inv, grad_color, grad_alpha, loss_accum);
+  else
+    hipLaunchKernelGGL(k_loss_mse<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream, color, alpha,
+                       bg, bg_value, target, npix, inv, grad_color, grad_alpha, loss_accum);
   LAUNCH_CHECK("k_loss_mse");
   return 0;
 }

@@ -13,6 +13,36 @@ namespace dbmp {
 // result = C + (1 - A) * bg;  r = result - target;  loss += sum(r^2) * inv_count
 // dL/dC = 2 r * inv_count;  dL/dA = -sum_c dL/dC_c * bg_c.
 // bg is (H,W,3) or, when null, the constant bg_value.
+__device__ __forceinline__ float loss_pixel(float a, const float* c, const float* b, float bg_value,
+                                            const float* t, float inv_count, float* g, float& acc) {
+  const float om = 1.f - a;
+  float ga = 0.f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float bk = b ? b[k] : bg_value;
+    const float r = fmaf(om, bk, c[k]) - t[k];
+    acc = fmaf(r, r, acc);
+    g[k] = 2.f * r * inv_count;
+    ga = fmaf(-g[k], bk, ga);
+  }
+  return ga;
+}
+
+#ifndef DBMP_LOSS_GROUPS
+#define DBMP_LOSS_GROUPS 4
+#endif
+// float4 groups (4 pixels each) a thread loads before it computes: the loads
+// of one iteration are independent, so a thread keeps G x 10 of them in
+// flight and the grid is G times smaller, which also divides the number of
+// blocks that end in an atomic add on the one loss value (same-address
+// atomics serialise)
+constexpr int kLossGroups = DBMP_LOSS_GROUPS;
+
+// kVec: every pointer 16-byte aligned; each thread takes kLossGroups float4
+// groups per iteration (group q: alpha[4q..4q+3] as one float4, colour /
+// target / bg as three), spaced one grid apart so every load is coalesced
+// across the wave; the npix % 4 tail pixels go through the scalar path.
+template <bool kVec>
 static __global__ __launch_bounds__(256) void k_loss_mse(const float* __restrict__ color,
                                                   const float* __restrict__ alpha,
                                                   const float* __restrict__ bg, float bg_value,
@@ -20,21 +50,57 @@ static __global__ __launch_bounds__(256) void k_loss_mse(const float* __restrict
                                                   float inv_count, float* __restrict__ gC,
                                                   float* __restrict__ gA, float* __restrict__ loss) {
   float acc = 0.f;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
-       p += (long long)gridDim.x * blockDim.x) {
-    const float om = 1.f - alpha[p];
-    float ga = 0.f;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nthr = (long long)gridDim.x * blockDim.x;
+  long long p0 = 0;
+  if (kVec) {
+    constexpr int G = kLossGroups;
+    const long long nq = npix >> 2;
+    for (long long base = tid; base < nq; base += G * nthr) {
+      float4 av[G], cv[G][3], tv[G][3], bv[G][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float b = bg ? bg[3 * p + c] : bg_value;
-      const float r = fmaf(om, b, color[3 * p + c]) - target[3 * p + c];
-      acc = fmaf(r, r, acc);
-      const float g = 2.f * r * inv_count;
-      gC[3 * p + c] = g;
-      ga = fmaf(-g, b, ga);
+      for (int k = 0; k < G; ++k) {
+        const long long q = base + k * nthr;
+        if (q < nq) {
+          av[k] = reinterpret_cast<const float4*>(alpha)[q];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            cv[k][j] = reinterpret_cast<const float4*>(color)[3 * q + j];
+            tv[k][j] = reinterpret_cast<const float4*>(target)[3 * q + j];
+            if (bg) bv[k][j] = reinterpret_cast<const float4*>(bg)[3 * q + j];
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const long long q = base + k * nthr;
+        if (q >= nq) break;
+        float c[12], t[12], b[12], g[12], ga[4];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          c[4 * j] = cv[k][j].x; c[4 * j + 1] = cv[k][j].y; c[4 * j + 2] = cv[k][j].z; c[4 * j + 3] = cv[k][j].w;
+          t[4 * j] = tv[k][j].x; t[4 * j + 1] = tv[k][j].y; t[4 * j + 2] = tv[k][j].z; t[4 * j + 3] = tv[k][j].w;
+          if (bg) {
+            b[4 * j] = bv[k][j].x; b[4 * j + 1] = bv[k][j].y; b[4 * j + 2] = bv[k][j].z; b[4 * j + 3] = bv[k][j].w;
+          }
+        }
+        const float a[4] = {av[k].x, av[k].y, av[k].z, av[k].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          ga[i] = loss_pixel(a[i], c + 3 * i, bg ? b + 3 * i : nullptr, bg_value, t + 3 * i, inv_count,
+                             g + 3 * i, acc);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          reinterpret_cast<float4*>(gC)[3 * q + j] =
+              make_float4(g[4 * j], g[4 * j + 1], g[4 * j + 2], g[4 * j + 3]);
+        reinterpret_cast<float4*>(gA)[q] = make_float4(ga[0], ga[1], ga[2], ga[3]);
+      }
     }
-    gA[p] = ga;
+    p0 = nq << 2;
   }
+  for (long long p = p0 + tid; p < npix; p += nthr)
+    gA[p] = loss_pixel(alpha[p], color + 3 * p, bg ? bg + 3 * p : nullptr, bg_value, target + 3 * p,
+                       inv_count, gC + 3 * p, acc);
   // wave then block reduction, one atomic per block
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);

@@ -41,7 +41,7 @@ __all__ = [
     "TileRasterizerBatchFunction", "CUDA_AVAILABLE", "CUDA_FP16_AVAILABLE",
     "print_cuda_timing_stats", "reset_cuda_timing_stats", "print_cuda_timing_stats_fp16",
     "reset_cuda_timing_stats_fp16", "DEFAULT_EPS_T", "bin_primitives", "pack_mapping",
-    "unpack_mapping",
+    "unpack_mapping", "CompositeMSELoss", "composite_mse_loss",
 ]
 
 DEFAULT_EPS_T = 1e-6
@@ -335,6 +335,48 @@ class TileRasterizerBatchFunction(Function):
         return tuple(g) + (None,) * 11
 
 
+class CompositeMSELoss(Function):
+    """``mean((color + (1 - alpha) * bg - target)^2)`` with its gradient, one
+    HIP kernel (``dbmp_loss_mse``) in place of the torch composite and
+    ``F.mse_loss`` the reference's renderer builds around the rasterizer
+    (``simple_tile_renderer.py:703-707``, ``loss_functions.py:20-40``) and their
+    autograd backward.  The gradient with respect to color and alpha is formed in
+    the forward pass and scaled by the incoming gradient in the backward;
+    ``target`` and ``bg`` are constants."""
+
+    @staticmethod
+    def forward(ctx, color, alpha, target, bg, handle, bg_value):
+        H, W = handle.H, handle.W
+        for t, nm, sh in ((color, "color", (H, W, 3)), (alpha, "alpha", (H, W)),
+                          (target, "target", (H, W, 3)), (bg, "bg", (H, W, 3))):
+            if t is not None and tuple(t.shape) != sh:
+                raise ValueError(f"{nm} must be {sh}, got {tuple(t.shape)}")
+        for t, nm in ((target, "target"), (bg, "bg")):
+            if t is not None and t.requires_grad:
+                raise ValueError(f"composite_mse_loss: {nm} is not differentiable here")
+        f32 = torch.float32
+        c, a = color.to(f32).contiguous(), alpha.to(f32).contiguous()
+        t = target.to(f32).contiguous()
+        b = None if bg is None else bg.to(f32).contiguous()
+        gC, gA = torch.empty_like(c), torch.empty_like(a)
+        loss = torch.zeros(1, dtype=f32, device=c.device)
+        handle.loss_mse(c, a, t, gC, gA, loss, bg=b, bg_value=bg_value)
+        ctx.save_for_backward(gC, gA)
+        return loss.view(())
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        gC, gA = ctx.saved_tensors
+        gC, gA = torch._foreach_mul((gC, gA), grad_loss)   # one multi-tensor kernel
+        return gC, gA, None, None, None, None
+
+
+def composite_mse_loss(handle: Handle, color, alpha, target, bg=None, bg_value: float = 1.0):
+    """Fused background composite + MSE loss (extension; see ``CompositeMSELoss``).
+    ``bg`` is an (H,W,3) image or None for the constant ``bg_value``."""
+    return CompositeMSELoss.apply(color, alpha, target, bg, handle, float(bg_value))
+
+
 class TileRasterizer:
     """Class-based rasterizer (``cuda_tile_rasterizer/__init__.py:60-121``)."""
 
@@ -415,6 +457,11 @@ class TileRasterizer:
 
     def enable_timing(self, on: bool = True) -> None:
         self.handle.timing_enable(on)
+
+    def composite_mse_loss(self, color, alpha, target, bg=None, bg_value: float = 1.0):
+        """Extension: ``F.mse_loss(color + (1 - alpha)[..., None] * bg, target)``
+        and its gradient in one kernel on this rasterizer's canvas."""
+        return composite_mse_loss(self.handle, color, alpha, target, bg, bg_value)
 
 
 def _print_stats(kind: str) -> None:

@@ -250,3 +250,50 @@ def test_c0_end_to_end_fit(dev):
         med, far = d.median().item(), (d > 0.25).float().mean().item()
         print(f"{k}: median |diff| {med:.2e}, fraction > 0.25: {far:.4f}, max {d.max().item():.3f}")
         assert med <= 5e-3 and far <= 0.02, k
+
+
+@pytest.mark.parametrize("with_bg", [False, True])
+def test_composite_mse_loss_matches_torch(dev, with_bg):
+    """The drop-in fused loss op equals torch's composite + F.mse_loss, value
+    and gradients, also through an fp16 rasterizer output and a scaled loss."""
+    from diffbmp_amd.tile_rasterizer import composite_mse_loss
+    g = torch.Generator(dev).manual_seed(11)
+    # 48x80: the float4 path; 7x9 (63 pixels): float4 groups plus a 3-pixel
+    # tail; the offset view: pointers off 16-byte alignment, the scalar path
+    for H, W, dt, offset in ((48, 80, torch.float32, 0), (48, 80, torch.float16, 0),
+                             (7, 9, torch.float32, 0), (48, 80, torch.float32, 1)):
+        h = Handle(H, W, 16)
+        target = torch.rand((H, W, 3), device=dev, generator=g)
+        bg = torch.rand((H, W, 3), device=dev, generator=g) if with_bg else None
+        if offset:
+            buf = torch.rand(H * W * 3 + offset, device=dev, generator=g)
+            c0 = buf[offset:].view(H, W, 3)
+            assert c0.data_ptr() % 16 != 0
+        else:
+            c0 = torch.rand((H, W, 3), device=dev, generator=g).to(dt)
+        a0 = torch.rand((H, W), device=dev, generator=g).to(dt)
+        c1, a1 = c0.clone().requires_grad_(), a0.clone().requires_grad_()
+        if offset:
+            c2 = buf.clone()[offset:].view(H, W, 3).requires_grad_()
+        else:
+            c2 = c0.clone().requires_grad_()
+        a2 = a0.clone().requires_grad_()
+        b = torch.ones_like(target) if bg is None else bg
+        ref = F.mse_loss(c1.float() + (1 - a1.float()).unsqueeze(-1) * b, target)
+        (3.0 * ref).backward()
+        got = composite_mse_loss(h, c2, a2, target, bg)
+        assert got.dim() == 0
+        (3.0 * got).backward()
+        assert abs(got.item() - ref.item()) <= 1e-5 * ref.item()
+        # the kernel forms the composite with one fused multiply-add, torch
+        # rounds the product and the sum: the residual may differ by ~1 ulp of
+        # the composite (<= 2, ulp 2.4e-7), the gradient by 2*3/n of that
+        ulp_g = 6.0 / (3 * H * W) * 2.4e-7
+        tol = dict(rtol=1e-5, atol=2 * ulp_g) if dt == torch.float32 else dict(rtol=2e-3, atol=1e-7)
+        assert c2.grad.dtype == dt and a2.grad.dtype == dt
+        assert torch.allclose(c2.grad.float(), c1.grad.float(), **tol)
+        assert torch.allclose(a2.grad.float(), a1.grad.float(), **tol)
+    with pytest.raises(ValueError):
+        composite_mse_loss(h, c2, a2, target[:-1])
+    with pytest.raises(ValueError):
+        composite_mse_loss(h, c2, a2, target.clone().requires_grad_())
