@@ -82,4 +82,6 @@ def test_precision_error_table(dev, store):
         # of opacity / colour logits and bitmap texels: within the FP16 bar
         assert t["image_max_abs"] <= 2e-3
     if store == "bf16-app":
-        assert t["image_max_abs"] <= 1.6e-2
+        # bf16 rounds each stored texel, opacity and colour logit by up to
+        # 2^-9 (~2e-3) relative; a pixel's image error stays within twice that
+        assert t["image_max_abs"] <= 4e-3
