@@ -80,17 +80,25 @@ template <> struct QuadT<kBF16> {
 // (fp16 / bf16) record: a sample's alpha and colour arrive in one cache line.
 template <int DT> struct RecT;
 template <> struct RecT<kF32> {
-  struct Col { float v[12]; };
+  // Only the record's address travels with the sample; its colour part is
+  // read when the colour is needed (colours()).  Carrying the twelve values
+  // instead let the compiler merge the two colour-map sources (this record,
+  // or a per-primitive map in global memory) into one load through a
+  // selected pointer, which put the record in private (scratch) memory.
+  struct Col { const float4* rec; };
   __device__ static __forceinline__ void get(const void* base, unsigned i, float& a, float& b,
                                              float& c, float& d, Col& col) {
     const float4* q = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (i << 6));
-    const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
+    const float4 r0 = q[0];
     a = r0.x; b = r0.y; c = r0.z; d = r0.w;
-    col.v[0] = r1.x; col.v[1] = r1.y; col.v[2] = r1.z; col.v[3] = r1.w;
-    col.v[4] = r2.x; col.v[5] = r2.y; col.v[6] = r2.z; col.v[7] = r2.w;
-    col.v[8] = r3.x; col.v[9] = r3.y; col.v[10] = r3.z; col.v[11] = r3.w;
+    col.rec = q;
   }
-  __device__ static __forceinline__ float at(const Col& col, int k) { return col.v[k]; }
+  __device__ static __forceinline__ void colours(const Col& col, float v[12]) {
+    const float4 r1 = col.rec[1], r2 = col.rec[2], r3 = col.rec[3];
+    v[0] = r1.x; v[1] = r1.y; v[2] = r1.z; v[3] = r1.w;
+    v[4] = r2.x; v[5] = r2.y; v[6] = r2.z; v[7] = r2.w;
+    v[8] = r3.x; v[9] = r3.y; v[10] = r3.z; v[11] = r3.w;
+  }
 };
 template <int DT>
 struct RecHalf {
@@ -104,8 +112,9 @@ struct RecHalf {
     a = lo(r0.x); b = hi(r0.x); c = lo(r0.y); d = hi(r0.y);
     col.v[0] = r0.z; col.v[1] = r0.w; col.v[2] = r1.x; col.v[3] = r1.y; col.v[4] = r1.z; col.v[5] = r1.w;
   }
-  __device__ static __forceinline__ float at(const Col& col, int k) {
-    return (k & 1) ? hi(col.v[k >> 1]) : lo(col.v[k >> 1]);
+  __device__ static __forceinline__ void colours(const Col& col, float v[12]) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = (k & 1) ? hi(col.v[k >> 1]) : lo(col.v[k >> 1]);
   }
 };
 template <> struct RecT<kF16> : RecHalf<kF16> {};

@@ -105,12 +105,14 @@ __device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const Prim
   k.fx = cx - (float)x0;
   k.fy = cy - (float)y0;
   if (A.cmap_mode == 2) {
+    float rc[12];
+    RecT<DT>::colours(raw, rc);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-      k.c00[ch] = RecT<DT>::at(raw, ch);
-      k.c10[ch] = RecT<DT>::at(raw, 3 + ch);
-      k.c01[ch] = RecT<DT>::at(raw, 6 + ch);
-      k.c11[ch] = RecT<DT>::at(raw, 9 + ch);
+      k.c00[ch] = rc[ch];
+      k.c10[ch] = rc[3 + ch];
+      k.c01[ch] = rc[6 + ch];
+      k.c11[ch] = rc[9 + ch];
     }
   } else {
     const StoreT<DT>* cm = (const StoreT<DT>*)A.cmap + (long)n * A.Ht * A.Wt * 3;
