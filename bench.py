@@ -399,6 +399,51 @@ def run_stress(args, dev):
     print(json.dumps(out))
 
 
+def stress_precision_summary(dev, steps=40, warmup=10, psnr_steps=200):
+    """The stress workload (2048^2, 32768 primitives, 16 RGBA 256^2 bitmaps,
+    c_blend 0.5) with fp32, fp16 and mixed storage: step rate, its ratio to
+    fp32, and the PSNR (dB) after `psnr_steps` Adam steps from one start.  Its
+    RGBA records (64 B per texel in fp32, 32 B in fp16) do not fit the L2, so
+    the storage width shows in the step time here."""
+    import torch
+    from diffbmp_amd import scenes as S
+    from diffbmp_amd.fit import FitJob
+    sc = S.make_scene(3)
+    co = torch.from_numpy(np.ascontiguousarray(sc.colors_orig_tmpl)).to(dev)
+    out = {}
+    for name, dt, geom32 in (("fp32", torch.float32, False), ("fp16", torch.float16, False),
+                             ("fp16-app", torch.float16, True)):
+        def new_job(n):
+            pr = make_problem(sc, dev, dt)
+            return pr, FitJob(*pr["params"], pr["templates"], pr["sel"], pr["target"], lr=1e-2,
+                              store_dtype=dt, fp32_geometry=geom32, max_steps=n + 1, colors_orig=co,
+                              c_blend=sc.c_blend, colors_orig_per_template=True)
+        _, job = new_job(warmup + steps)
+        for _ in range(warmup):
+            job.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            job.step()
+        torch.cuda.synchronize()
+        rate = steps / (time.perf_counter() - t0)
+        del job
+        pr, job = new_job(psnr_steps)
+        for _ in range(psnr_steps):
+            job.step()
+        img = job.render()
+        mse = float(((img - pr["target"]) ** 2).mean().item())
+        out[name] = {"steps_per_s": round(rate, 2), "psnr_db": round(FitJob.psnr(mse), 4)}
+        del job, pr
+    for name in out:
+        out[name]["speed_vs_fp32"] = round(out[name]["steps_per_s"] / out["fp32"]["steps_per_s"], 4)
+        out[name]["psnr_minus_fp32_db"] = round(out[name]["psnr_db"] - out["fp32"]["psnr_db"], 4)
+    out["psnr_steps"] = psnr_steps
+    out["workload"] = ("stress: 2048x2048, 32768 primitives, 16x256^2 RGBA bitmaps (per-template "
+                       "colour maps, c_blend 0.5), Adam lr 1e-2 on MSE")
+    return out
+
+
 def serving_summary(dev, jobs=40, rate=10.0, steps=200, seed=2025):
     """A bounded serving run for the headline line: ``jobs`` fit jobs of the
     serving mix arriving as a Poisson stream of ``rate`` jobs/s."""
@@ -735,6 +780,7 @@ def main():
         out["psnr"] = psnr_compare(args, sc, dev, sdt)
     if not args.no_precision:
         out["storage_precision"] = precision_summary(sc, dev)
+        out["stress_storage_precision"] = stress_precision_summary(dev)
     out["plain_fit"] = gpu_plain_fit(dev)
     if not args.no_serving:
         out["serving"] = serving_summary(dev)
