@@ -117,24 +117,45 @@ def template_library(size: int = 256, count: int = 16, kind: str = "stress"):
     return _LIBRARY[key]
 
 
+_DEVICE_LIBRARY: dict = {}
+
+
+def device_library(size: int, kind: str, dtype, device):
+    """``template_library`` on the device in ``dtype``, uploaded once per
+    (size, kind, dtype, device); jobs select their bitmaps from it."""
+    key = (size, kind, dtype, str(device))
+    if key not in _DEVICE_LIBRARY:
+        tm, cm = template_library(size, kind=kind)
+        _DEVICE_LIBRARY[key] = (torch.from_numpy(tm).to(device).to(dtype).contiguous(),
+                                torch.from_numpy(cm).to(device).to(dtype).contiguous())
+    return _DEVICE_LIBRARY[key]
+
+
 def smooth_target(H: int, W: int, seed: int, device, terms: int = 32) -> torch.Tensor:
     """``clamp(0.5 + sum_i (a_i/8) sin(2 pi (kx_i x/W + ky_i y/H) + phi_i), 0, 1)``
-    per channel (a ~ U[0,1], k ~ U[0.5,8], phi ~ U[0,2 pi)), evaluated on the GPU."""
+    per channel (a ~ U[0,1], k ~ U[0.5,8], phi ~ U[0,2 pi)), evaluated on the GPU.
+
+    Each term separates, sin(p(x) + q(y)) = sin p cos q + cos p sin q, so a
+    channel is 0.5 + U V with U (H, 2 terms) the row factors and V (2 terms, W)
+    the column factors: one batched product for the three channels instead of
+    a few hundred elementwise kernels."""
     rng = np.random.default_rng(seed)
-    yy = torch.arange(H, device=device, dtype=torch.float32)[:, None] / H
-    xx = torch.arange(W, device=device, dtype=torch.float32)[None, :] / W
-    out = torch.empty((H, W, 3), device=device, dtype=torch.float32)
+    coef = np.empty((4, 3, terms), np.float64)   # a, kx, ky, phi per channel, in draw order
     for ch in range(3):
-        a = rng.uniform(0, 1, terms)
-        kx = rng.uniform(0.5, 8, terms)
-        ky = rng.uniform(0.5, 8, terms)
-        ph = rng.uniform(0, 2 * math.pi, terms)
-        f = torch.full((H, W), 0.5, device=device)
-        for i in range(terms):
-            f += float(a[i] / 8.0) * torch.sin(2 * math.pi * (float(kx[i]) * xx + float(ky[i]) * yy)
-                                               + float(ph[i]))
-        out[..., ch] = f.clamp_(0, 1)
-    return out
+        coef[0, ch] = rng.uniform(0, 1, terms)
+        coef[1, ch] = rng.uniform(0.5, 8, terms)
+        coef[2, ch] = rng.uniform(0.5, 8, terms)
+        coef[3, ch] = rng.uniform(0, 2 * math.pi, terms)
+    a, kx, ky, ph = torch.from_numpy(coef.astype(np.float32)).to(device).unbind(0)   # (3, terms)
+    y = torch.arange(H, device=device, dtype=torch.float32) / H
+    x = torch.arange(W, device=device, dtype=torch.float32) / W
+    q = (2 * math.pi) * ky[:, None, :] * y[None, :, None] + ph[:, None, :]          # (3, H, terms)
+    p = (2 * math.pi) * kx[:, :, None] * x[None, None, :]                           # (3, terms, W)
+    a8 = (a / 8.0)[:, None, :]
+    U = torch.cat((a8 * torch.cos(q), a8 * torch.sin(q)), dim=2)                    # (3, H, 2 terms)
+    V = torch.cat((torch.sin(p), torch.cos(p)), dim=1)                              # (3, 2 terms, W)
+    f = torch.bmm(U, V).add_(0.5).clamp_(0, 1)                                      # (3, H, W)
+    return f.permute(1, 2, 0).contiguous()
 
 
 def random_spec(rng: np.random.Generator, steps: int = 200, library_size: int = 16,
@@ -156,12 +177,12 @@ class FitState:
         self.spec = spec
         H, W, N = spec.height, spec.width, spec.num_primitives
         rng = np.random.default_rng(spec.seed)
-        lib_t, lib_c = template_library(spec.template_size, kind=spec.library)
-        ids = list(spec.template_ids)
-        T = len(ids)
+        lib_t, lib_c = device_library(spec.template_size, spec.library, sdt, dev)
+        ids = torch.tensor(list(spec.template_ids), dtype=torch.long, device=dev)
+        T = int(ids.numel())
         Ht, Wt = lib_t.shape[1:]
-        self.templates = torch.from_numpy(lib_t[ids]).to(dev).to(sdt).contiguous()
-        self.colors_orig = torch.from_numpy(lib_c[ids]).to(dev).to(sdt).contiguous()
+        self.templates = lib_t.index_select(0, ids)
+        self.colors_orig = lib_c.index_select(0, ids)
         self.ext = torch.empty((T, 4), dtype=torch.float32, device=dev)
         self.blend = spec.c_blend > 0
         rec = _RGBA_BYTES[sdt] if self.blend else _QUAD_BYTES[sdt]
