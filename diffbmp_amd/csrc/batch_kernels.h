@@ -76,9 +76,9 @@ static __global__ __launch_bounds__(64) void k_batch_cull(const BatchJob* __rest
   float4* out = const_cast<float4*>(d.A.cull);
   const float du = 2.f / (float)(d.A.Wt - 1), dv = 2.f / (float)(d.A.Ht - 1);
   for (int t = threadIdx.x; t < d.T; t += 64) {
-    const float* e = d.ext + 4 * t;
+    const DBMP_GLOBAL float* e = gptr(d.ext) + 4 * t;
     const float u0 = e[0] - du, u1 = e[1] + du, v0 = e[2] - dv, v1 = e[3] + dv;
-    out[t] = make_float4(0.5f * (u0 + u1), 0.5f * (v0 + v1), 0.5f * (u1 - u0), 0.5f * (v1 - v0));
+    st_f4(out + t, make_float4(0.5f * (u0 + u1), 0.5f * (v0 + v1), 0.5f * (u1 - u0), 0.5f * (v1 - v0)));
   }
 }
 
@@ -95,8 +95,8 @@ static __global__ __launch_bounds__(256) void k_batch_prep(const BatchJob* __res
   const int n = g - prim_start[j];
   float q[8];
   if constexpr (DT == kF32) {
-    const float4* s = reinterpret_cast<const float4*>(B.store) + 2L * n;
-    const float4 a = s[0], b = s[1];
+    const DBMP_GLOBAL char* s = gbytes(B.store) + 32L * n;
+    const float4 a = ld_f4(s), b = ld_f4(s + 16);
     q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w; q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
   } else {
     using ST = typename std::conditional<DT == kF16, __half, __hip_bfloat16>::type;
@@ -108,7 +108,7 @@ static __global__ __launch_bounds__(256) void k_batch_prep(const BatchJob* __res
   int cnt;
   int4 rg;
   const RasterArgs& A = B.A;
-  prep_core(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], B.sel[n], B.ext, B.T, A.Ht, A.Wt, A.H,
+  prep_core(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], gptr(B.sel)[n], B.ext, B.T, A.Ht, A.Wt, A.H,
             A.W, kSub, B.tilesX16, B.tilesY16, B.alpha_ub, A.bin_shift, p, cnt, rg);
   recs[g] = p;
   counts[g] = cnt;
@@ -177,7 +177,9 @@ __device__ __forceinline__ int next_item(const WorkQueue& q, int* s_item) {
     *s_item = item;
   }
   __syncthreads();
-  const int item = *s_item;
+  // uniform across the workgroup: readfirstlane tells the compiler, so the
+  // job lookup and the job's descriptor stay in scalar registers
+  const int item = __builtin_amdgcn_readfirstlane(*s_item);
   __syncthreads();   // every wave has read it before thread 0 writes the next
   return item;
 }
@@ -258,17 +260,18 @@ static __global__ __launch_bounds__(256) void k_batch_adam(const BatchJob* __res
   const long long k = 8LL * (g - prim_start[j]) + q;
   const int grp = q < 2 ? 0 : (q < 5 ? q - 1 : 4);
   const float gr = grad8[i];
-  const float m = fmaf(beta1, B.m[k], (1.f - beta1) * gr);
-  const float v = fmaf(beta2, B.v[k], (1.f - beta2) * (gr * gr));
-  B.m[k] = m;
-  B.v[k] = v;
-  float p = B.p[k] - state[j].step_scale[grp] * (m / (sqrtf(v) * state[j].inv_bc2_sqrt + eps));
+  const float m = fmaf(beta1, gptr(B.m)[k], (1.f - beta1) * gr);
+  const float v = fmaf(beta2, gptr(B.v)[k], (1.f - beta2) * (gr * gr));
+  gptr(B.m)[k] = m;
+  gptr(B.v)[k] = v;
+  float p = gptr(B.p)[k] - state[j].step_scale[grp] * (m / (sqrtf(v) * state[j].inv_bc2_sqrt + eps));
   if (q < 3 && B.lo[q] <= B.hi[q]) p = fminf(fmaxf(p, B.lo[q]), B.hi[q]);
-  B.p[k] = p;
+  gptr(B.p)[k] = p;
   if (B.store_out) {
-    if constexpr (DT == kF16) ((__half*)B.store_out)[k] = __float2half(p);
-    else if constexpr (DT == kBF16) ((__hip_bfloat16*)B.store_out)[k] = __float2bfloat16(p);
-    else ((float*)B.store_out)[k] = p;
+    DBMP_GLOBAL unsigned short* h = (DBMP_GLOBAL unsigned short*)B.store_out;
+    if constexpr (DT == kF16) h[k] = __half_as_ushort(__float2half(p));
+    else if constexpr (DT == kBF16) h[k] = __bfloat16_as_ushort(__float2bfloat16(p));
+    else gptr((float*)B.store_out)[k] = p;
   }
 }
 

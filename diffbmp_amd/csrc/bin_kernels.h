@@ -147,7 +147,7 @@ __device__ __forceinline__ void prep_core(float x, float y, float r, float th, f
   cnt = 0;
   rg = make_int4(0, -1, 0, -1);
   if (valid && ext != nullptr) {
-    const float* e = ext + 4 * t;
+    const DBMP_GLOBAL float* e = gptr(ext) + 4 * t;
     const float cu[4] = {e[0], e[1], e[0], e[1]};
     const float cv[4] = {e[2], e[2], e[3], e[3]};
     float xmn = 0.f, xmx = 0.f, ymn = 0.f, ymx = 0.f;

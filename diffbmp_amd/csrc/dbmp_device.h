@@ -30,10 +30,62 @@ struct alignas(16) PrimRec {
 };
 static_assert(sizeof(PrimRec) == 64, "PrimRec is 64 bytes");
 
+// Global-memory access ------------------------------------------------------
+// A pointer that a kernel reads out of a descriptor in memory (the batched
+// job table) is generic to the compiler, which then emits flat loads and
+// stores.  Those count against the LDS wait counter as well as the vector
+// memory one, so each wait for an LDS read also waits for them.
+// Dereferencing through an address-space-1 (global) pointer gives global
+// instructions; for kernel arguments, already known to be global, it
+// changes nothing.  Every buffer these kernels touch is device memory.
+#define DBMP_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const DBMP_GLOBAL T* gptr(const T* p) { return (const DBMP_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ DBMP_GLOBAL T* gptr(T* p) { return (DBMP_GLOBAL T*)p; }
+__device__ __forceinline__ const DBMP_GLOBAL char* gbytes(const void* p) { return (const DBMP_GLOBAL char*)p; }
+// 16-byte / 8-byte loads and stores through global pointers.  HIP's float4,
+// uint2 and uint4 have user-defined constructors, which cannot copy from an
+// address-space-qualified object; clang's native vectors can.
+typedef float dbmp_f4 __attribute__((ext_vector_type(4)));
+typedef int dbmp_i4 __attribute__((ext_vector_type(4)));
+typedef unsigned dbmp_u2 __attribute__((ext_vector_type(2)));
+typedef unsigned dbmp_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_f4(const DBMP_GLOBAL char* p) {
+  const dbmp_f4 v = *(const DBMP_GLOBAL dbmp_f4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 ld_i4(const DBMP_GLOBAL char* p) {
+  const dbmp_i4 v = *(const DBMP_GLOBAL dbmp_i4*)p;
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ld_u2(const DBMP_GLOBAL char* p) {
+  const dbmp_u2 v = *(const DBMP_GLOBAL dbmp_u2*)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 ld_u4(const DBMP_GLOBAL char* p) {
+  const dbmp_u4 v = *(const DBMP_GLOBAL dbmp_u4*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_f4(void* p, float4 v) {
+  dbmp_f4 w;
+  w.x = v.x; w.y = v.y; w.z = v.z; w.w = v.w;
+  *(DBMP_GLOBAL dbmp_f4*)p = w;
+}
+
+// hardware fp32 atomic add (relaxed, device scope) on global memory
+__device__ __forceinline__ void atomic_add_global(float* p, float v) {
+  __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Storage-type loads -------------------------------------------------------
-__device__ __forceinline__ float ldf(const float* p, long i) { return p[i]; }
-__device__ __forceinline__ float ldf(const __half* p, long i) { return __half2float(p[i]); }
-__device__ __forceinline__ float ldf(const __hip_bfloat16* p, long i) { return __bfloat162float(p[i]); }
+__device__ __forceinline__ float ldf(const float* p, long i) { return gptr(p)[i]; }
+__device__ __forceinline__ float ldf(const __half* p, long i) {
+  return __half2float(__ushort_as_half(((const DBMP_GLOBAL unsigned short*)p)[i]));
+}
+__device__ __forceinline__ float ldf(const __hip_bfloat16* p, long i) {
+  return __uint_as_float((uint32_t)((const DBMP_GLOBAL unsigned short*)p)[i] << 16);   // exact
+}
 
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
@@ -53,21 +105,21 @@ template <> struct QuadT<kF32> {
   using type = float4;
   // i < 2^28 (the byte offset fits 32 bits: one base register + a 32-bit offset)
   __device__ static __forceinline__ void get(const float4* q, unsigned i, float& a, float& b, float& c, float& d) {
-    const float4 v = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q) + (i << 4));
+    const float4 v = ld_f4(gbytes(q) + (i << 4));
     a = v.x; b = v.y; c = v.z; d = v.w;
   }
 };
 template <> struct QuadT<kF16> {
   using type = uint2;
   __device__ static __forceinline__ void get(const uint2* q, unsigned i, float& a, float& b, float& c, float& d) {
-    const uint2 v = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(q) + (i << 3));
+    const uint2 v = ld_u2(gbytes(q) + (i << 3));
     a = f16_lo(v.x); b = f16_hi(v.x); c = f16_lo(v.y); d = f16_hi(v.y);
   }
 };
 template <> struct QuadT<kBF16> {
   using type = uint2;
   __device__ static __forceinline__ void get(const uint2* q, unsigned i, float& a, float& b, float& c, float& d) {
-    const uint2 v = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(q) + (i << 3));
+    const uint2 v = ld_u2(gbytes(q) + (i << 3));
     a = bf16_lo(v.x); b = bf16_hi(v.x); c = bf16_lo(v.y); d = bf16_hi(v.y);
   }
 };
@@ -85,16 +137,16 @@ template <> struct RecT<kF32> {
   // instead let the compiler merge the two colour-map sources (this record,
   // or a per-primitive map in global memory) into one load through a
   // selected pointer, which put the record in private (scratch) memory.
-  struct Col { const float4* rec; };
+  struct Col { const DBMP_GLOBAL char* rec; };
   __device__ static __forceinline__ void get(const void* base, unsigned i, float& a, float& b,
                                              float& c, float& d, Col& col) {
-    const float4* q = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (i << 6));
-    const float4 r0 = q[0];
+    const DBMP_GLOBAL char* q = gbytes(base) + (i << 6);
+    const float4 r0 = ld_f4(q);
     a = r0.x; b = r0.y; c = r0.z; d = r0.w;
     col.rec = q;
   }
   __device__ static __forceinline__ void colours(const Col& col, float v[12]) {
-    const float4 r1 = col.rec[1], r2 = col.rec[2], r3 = col.rec[3];
+    const float4 r1 = ld_f4(col.rec + 16), r2 = ld_f4(col.rec + 32), r3 = ld_f4(col.rec + 48);
     v[0] = r1.x; v[1] = r1.y; v[2] = r1.z; v[3] = r1.w;
     v[4] = r2.x; v[5] = r2.y; v[6] = r2.z; v[7] = r2.w;
     v[8] = r3.x; v[9] = r3.y; v[10] = r3.z; v[11] = r3.w;
@@ -107,8 +159,8 @@ struct RecHalf {
   __device__ static __forceinline__ float hi(uint32_t v) { return DT == kF16 ? f16_hi(v) : bf16_hi(v); }
   __device__ static __forceinline__ void get(const void* base, unsigned i, float& a, float& b,
                                              float& c, float& d, Col& col) {
-    const uint4* q = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(base) + (i << 5));
-    const uint4 r0 = q[0], r1 = q[1];
+    const DBMP_GLOBAL char* q = gbytes(base) + (i << 5);
+    const uint4 r0 = ld_u4(q), r1 = ld_u4(q + 16);
     a = lo(r0.x); b = hi(r0.x); c = lo(r0.y); d = hi(r0.y);
     col.v[0] = r0.z; col.v[1] = r0.w; col.v[2] = r1.x; col.v[3] = r1.y; col.v[4] = r1.z; col.v[5] = r1.w;
   }

@@ -68,7 +68,7 @@ struct RasterArgs {
 // costs when there is one, else the XCD-interleaved natural order.
 __device__ __forceinline__ int xcd_remap(int b, int nblocks);
 __device__ __forceinline__ int block_of(const RasterArgs& A, int wg) {
-  return A.order != nullptr ? A.order[wg] : xcd_remap(wg, A.tiles * A.subs);
+  return A.order != nullptr ? gptr(A.order)[wg] : xcd_remap(wg, A.tiles * A.subs);
 }
 
 // blockIdx -> (tile, sub-tile).  Consecutive workgroups are dealt round-robin
@@ -211,13 +211,13 @@ __device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A, int b) {
 // List range of a tile.  If binning overflowed its capacity the lists were
 // not written: walk all primitives in index order (same result, slower).
 __device__ __forceinline__ bool list_range(const RasterArgs& A, int tile, int& start, int& end) {
-  if (A.pcount != nullptr && *A.pcount > A.pcap) {
+  if (A.pcount != nullptr && *gptr(A.pcount) > A.pcap) {
     start = 0;
     end = A.N;
     return true;
   }
-  start = A.toffs[tile];
-  end = A.toffs[tile + 1];
+  start = gptr(A.toffs)[tile];
+  end = gptr(A.toffs)[tile + 1];
   return false;
 }
 
@@ -225,10 +225,10 @@ __device__ __forceinline__ bool list_range(const RasterArgs& A, int tile, int& s
 // quarter t&3 of record t>>2); entries past the end get an empty push box.
 __device__ __forceinline__ float4 fetch_rec_part(const RasterArgs& A, bool all, int k, int end,
                                                  int part, int& n_out) {
-  const int n = (k < end) ? (all ? k : A.tidx[k]) : -1;
+  const int n = (k < end) ? (all ? k : gptr(A.tidx)[k]) : -1;
   const bool ok = (unsigned)n < (unsigned)A.N;
   n_out = ok ? n : -1;
-  if (ok) return reinterpret_cast<const float4*>(A.recs + n)[part];
+  if (ok) return ld_f4(gbytes(A.recs + n) + 16 * part);
   return part == 0 ? make_float4(0.f, 0.f, -1.f, -1.f) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -266,7 +266,7 @@ __device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
 __device__ __forceinline__ bool content_hits(const RasterArgs& A, const PrimRec& p, float cx,
                                              float cy) {
   const int t = p.tmpl;
-  const float4 b = A.cull[min(max(t, 0), A.ntmpl - 1)];
+  const float4 b = ld_f4(gbytes(A.cull + min(max(t, 0), A.ntmpl - 1)));
   const float ac = fabsf(p.c), as = fabsf(p.s);
   const float dx = cx - p.mx, dy = cy - p.my;
   const float pu = fmaf(p.c, dx, p.s * dy) - p.r * b.x;
@@ -307,7 +307,7 @@ __device__ __forceinline__ Stage stage_load(const RasterArgs& A, bool all, int k
   // range is read where it decides: super-tile lists, and the walk over every
   // primitive after a binning overflow
   if (A.ranges != nullptr && part == 0 && st.n >= 0 && (all || A.bin_shift > 0))
-    st.rg = A.ranges[st.n];
+    st.rg = ld_i4(gbytes(A.ranges + st.n));
   return st;
 }
 __device__ __forceinline__ void stage_store(const Stage& st, PrimRec (*srec)[kBatch], int (*sid)[kBatch],
@@ -394,7 +394,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   __shared__ __align__(16) int salive[2][4];
   const BlockGeom G = block_geom(A, b);
   if (G.bx >= A.W || G.by >= A.H) {   // 16-px block of a tile past the canvas edge
-    if (A.cost != nullptr && threadIdx.x == 0) A.cost[b] = 0;
+    if (A.cost != nullptr && threadIdx.x == 0) gptr(A.cost)[b] = 0;
     return;
   }
   const float fpx = (float)G.px, fpy = (float)G.py;
@@ -498,12 +498,12 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   tally.flush(0);
   if (G.px < A.W && G.py < A.H) {
     const long p = (long)G.py * A.W + G.px;
-    out_color[3 * p + 0] = nz ? Cf0 : C0;
-    out_color[3 * p + 1] = nz ? Cf1 : C1;
-    out_color[3 * p + 2] = nz ? Cf2 : C2;
-    out_alpha[p] = 1.f - (nz ? 0.f : Tnz);
-    out_T[p] = Tnz;
-    out_last[p] = (last < 0 ? end - start : last) | (nz << 30);
+    gptr(out_color)[3 * p + 0] = nz ? Cf0 : C0;
+    gptr(out_color)[3 * p + 1] = nz ? Cf1 : C1;
+    gptr(out_color)[3 * p + 2] = nz ? Cf2 : C2;
+    gptr(out_alpha)[p] = 1.f - (nz ? 0.f : Tnz);
+    gptr(out_T)[p] = Tnz;
+    gptr(out_last)[p] = (last < 0 ? end - start : last) | (nz << 30);
   }
   if (A.cost != nullptr) {
     // block cost: the slowest wave's entry count plus the staging of its batches
@@ -511,7 +511,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     if (G.lane == 0) smask[0][G.w] = walked + 8 * batches;
     __syncthreads();
     if (threadIdx.x == 0)
-      A.cost[b] = max(max(smask[0][0], smask[0][1]), max(smask[0][2], smask[0][3]));
+      gptr(A.cost)[b] = max(max(smask[0][0], smask[0][1]), max(smask[0][2], smask[0][3]));
   }
 }
 
@@ -561,12 +561,12 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   const float fpx = (float)G.px, fpy = (float)G.py;
   const bool okpix = (G.px < A.W) && (G.py < A.H);
   const long pix = okpix ? (long)G.py * A.W + G.px : 0;
-  const int lstw = okpix ? lastin[pix] : 0;
+  const int lstw = okpix ? gptr(lastin)[pix] : 0;
   const int lst = lstw & 0x3fffffff;   // list entries consumed by the forward
   const int nzf = (unsigned)lstw >> 30; // exactly opaque layers seen (capped at 2)
-  const float cf0 = okpix ? Cfin[3 * pix] : 0.f, cf1 = okpix ? Cfin[3 * pix + 1] : 0.f,
-              cf2 = okpix ? Cfin[3 * pix + 2] : 0.f;
-  const float tnzf = okpix ? Tfin[pix] : 0.f;   // product of the nonzero (1 - a) factors
+  const float cf0 = okpix ? gptr(Cfin)[3 * pix] : 0.f, cf1 = okpix ? gptr(Cfin)[3 * pix + 1] : 0.f,
+              cf2 = okpix ? gptr(Cfin)[3 * pix + 2] : 0.f;
+  const float tnzf = okpix ? gptr(Tfin)[pix] : 0.f;   // product of the nonzero (1 - a) factors
   const float tf = nzf ? 0.f : tnzf;            // transmittance after the last layer
   float gc0, gc1, gc2, ga;
   float acc_loss = 0.f;   // this lane's squared error (LOSS)
@@ -578,18 +578,18 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     ga = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float bgc = L.bg ? (okpix ? L.bg[3 * pix + c] : 0.f) : L.bg_value;
-      const float r = okpix ? fmaf(om, bgc, cf[c]) - L.target[3 * pix + c] : 0.f;
+      const float bgc = L.bg ? (okpix ? gptr(L.bg)[3 * pix + c] : 0.f) : L.bg_value;
+      const float r = okpix ? fmaf(om, bgc, cf[c]) - gptr(L.target)[3 * pix + c] : 0.f;
       acc_loss = fmaf(r, r, acc_loss);
       g[c] = 2.f * r * L.inv_count;
       ga = fmaf(-g[c], bgc, ga);
     }
     gc0 = g[0]; gc1 = g[1]; gc2 = g[2];
   } else {
-    gc0 = okpix ? gCin[3 * pix] : 0.f;
-    gc1 = okpix ? gCin[3 * pix + 1] : 0.f;
-    gc2 = okpix ? gCin[3 * pix + 2] : 0.f;
-    ga = (okpix && gAin) ? gAin[pix] : 0.f;
+    gc0 = okpix ? gptr(gCin)[3 * pix] : 0.f;
+    gc1 = okpix ? gptr(gCin)[3 * pix + 1] : 0.f;
+    gc2 = okpix ? gptr(gCin)[3 * pix + 2] : 0.f;
+    ga = (okpix && gAin) ? gptr(gAin)[pix] : 0.f;
   }
   float T = 1.f, ca0 = 0.f, ca1 = 0.f, ca2 = 0.f;
   const float gcf = fmaf(gc2, cf2, fmaf(gc1, cf1, gc0 * cf0));   // gc . C_final
@@ -727,7 +727,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       sred[2][j][q] = 0.f;
       sred[3][j][q] = 0.f;
       const int n = sid[buf][j];
-      if (s != 0.f && n >= 0) unsafeAtomicAdd(&grad8[(long)n * 8 + q], s);
+      if (s != 0.f && n >= 0) atomic_add_global(&grad8[(long)n * 8 + q], s);
     }
     __syncthreads();
   }
@@ -741,7 +741,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
     if (G.lane == 0) sred[0][0][G.w] = acc_loss;
     __syncthreads();
     if (threadIdx.x == 0 && L.loss)
-      unsafeAtomicAdd(L.loss, (sred[0][0][0] + sred[0][0][1] + sred[0][0][2] + sred[0][0][3]) *
+      atomic_add_global(L.loss, (sred[0][0][0] + sred[0][0][1] + sred[0][0][2] + sred[0][0][3]) *
                                   L.inv_count);
   }
 }
