@@ -271,13 +271,14 @@ class Handle:
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.H, self.W, self.ts = int(image_height), int(image_width), int(tile_size)
         self.alpha_ub = float(alpha_upper_bound)
-        self.tilesX = -(-self.W // self.ts)
-        self.tilesY = -(-self.H // self.ts)
-        self.tiles = self.tilesX * self.tilesY
         h = ctypes.c_void_p()
+        # dbmp_init validates the sizes (and raises) before anything is derived from them
         with torch.cuda.device(self.device):
             check(L.dbmp_init(ctypes.byref(h), self.H, self.W, self.ts, float(sigma), self.alpha_ub,
                               int(max_prims_per_pixel), int(num_primitives or 0)))
+        self.tilesX = -(-self.W // self.ts)
+        self.tilesY = -(-self.H // self.ts)
+        self.tiles = self.tilesX * self.tilesY
         self._h = h
         self._L = L
         self.pair_capacity = 0

@@ -808,7 +808,9 @@ int dbmp_backward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
   if (rc) return rc;
   if ((rc = check_bitmaps(bm))) return rc;
   if (!grad_color || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
-  if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
+  // zero primitives: the gradient outputs are empty (and may be NULL)
+  if (prims[0].num_primitives > 0 &&
+      (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors))
     return fail(DBMP_E_ARG, "NULL gradient output");
   hipStream_t st = (hipStream_t)stream;
   const int N = prims[0].num_primitives;
@@ -863,7 +865,9 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   if ((rc = check_bitmaps(bm))) return rc;
   if ((rc = check_lists(h, lists))) return rc;
   if (!grad_color || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
-  if (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors)
+  // zero primitives: the gradient outputs are empty (and may be NULL)
+  if (prims->num_primitives > 0 &&
+      (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors))
     return fail(DBMP_E_ARG, "NULL gradient output");
   hipStream_t st = (hipStream_t)stream;
   const int N = prims->num_primitives;
@@ -904,7 +908,8 @@ static int backward_mse_impl(dbmp_rasterizer* h, const dbmp_prims* prims, const 
   if ((rc = check_bitmaps(bm))) return rc;
   if ((rc = check_lists(h, lists))) return rc;
   if (!target || !out_color || !out_T_final || !out_last) return fail(DBMP_E_ARG, "NULL input");
-  if (!rows && (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors))
+  if (prims->num_primitives > 0 && !rows &&
+      (!grad_means2D || !grad_radii || !grad_rotations || !grad_opacities || !grad_colors))
     return fail(DBMP_E_ARG, "NULL gradient output");
   const int N = prims->num_primitives;
   if ((rc = ensure_n(h, std::max(N, 1)))) return rc;
