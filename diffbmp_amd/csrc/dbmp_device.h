@@ -182,38 +182,6 @@ __device__ __forceinline__ float sigmoid_safe(float x) {
   return z / (1.f + z);
 }
 
-// Per-sample geometry in exact fp32 (tile_forward.cu:187-267).  Returns
-// false when the pixel is outside the push box or the texture coordinates
-// fall outside [0, W-1] x [0, H-1] (bilinear_sample returns 0 there, and a
-// zero mask is never pushed), or are NaN (undefined in the reference).
-struct Geo {
-  float u, v, ndx, ndy, tx, ty, wx, wy;
-  int x0, y0;
-};
-
-__device__ __forceinline__ bool sample_geom(const PrimRec& p, float px, float py, float Wtm1,
-                                            float Htm1, float hsx, float hsy, int Wt, int Ht,
-                                            Geo& g) {
-  const float dx = px - p.mx, dy = py - p.my;
-  if (fabsf(dx) > p.hw || fabsf(dy) > p.hh) return false;
-  const float ndx = dx * p.rinv, ndy = dy * p.rinv;
-  const float u = p.c * ndx + p.s * ndy;
-  const float v = -p.s * ndx + p.c * ndy;
-  // (u + 1) * 0.5 * (W-1) == (u + 1) * (0.5 * (W-1)) bit for bit: the 0.5
-  // scaling is exact on both sides.
-  const float tx = (u + 1.f) * hsx;
-  const float ty = (v + 1.f) * hsy;
-  // written so that NaN coordinates are rejected as well
-  if (!(tx >= 0.f && tx <= Wtm1 && ty >= 0.f && ty <= Htm1)) return false;
-  const float fx = floorf(tx), fy = floorf(ty);
-  g.x0 = (int)fx;   // in [0, Wt-1] after the test above
-  g.y0 = (int)fy;
-  g.wx = tx - fx;
-  g.wy = ty - fy;
-  g.u = u; g.v = v; g.ndx = ndx; g.ndy = ndy; g.tx = tx; g.ty = ty;
-  return true;
-}
-
 // Bilinear value and texel-space gradient from a quad (tile_common.h:201-239)
 __device__ __forceinline__ float bilerp(float v00, float v01, float v10, float v11, float wx,
                                         float wy) {

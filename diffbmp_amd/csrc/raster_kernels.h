@@ -325,8 +325,8 @@ __device__ __forceinline__ void stage_store(const Stage& st, PrimRec (*srec)[kBa
   }
 }
 
-// One sample, branch-free: geometry with the exact fp32 decisions of
-// sample_geom, one quad gather (a rejected lane gathers quad 0), bilinear
+// One sample, branch-free: geometry with the fp32 decisions of the oracle
+// (tile_forward.cu:187-267), one quad gather (a rejected lane gathers quad 0), bilinear
 // mask.  `pushed` is the reference's push condition (box, texture bounds,
 // mask > 1e-6) and the caller's activity flag.
 template <int DT>
@@ -346,8 +346,12 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   const bool inbox = (fabsf(dx) <= p.hw) & (fabsf(dy) <= p.hh);
   r.ndx = dx * p.rinv;
   r.ndy = dy * p.rinv;
-  r.u = p.c * r.ndx + p.s * r.ndy;
-  r.v = -p.s * r.ndx + p.c * r.ndy;
+  // u = c ndx + s ndy and v = -s ndx + c ndy with the first product fused
+  // into the add, as nvcc's default contraction (-fmad=true) compiles the
+  // reference's expressions (tile_forward.cu:259-260); the oracle uses the
+  // same fused form
+  r.u = fmaf(p.c, r.ndx, p.s * r.ndy);
+  r.v = fmaf(-p.s, r.ndx, p.c * r.ndy);
   r.tx = (r.u + 1.f) * A.hsx;
   r.ty = (r.v + 1.f) * A.hsy;
   const bool inb = (r.tx >= 0.f) & (r.tx <= (float)(A.Wt - 1)) & (r.ty >= 0.f) &

@@ -216,8 +216,10 @@ static int sample_prim(const dbmp_oracle_scene *s, int n, int px, int py, sample
         const float phi = s->rotations[n];
         const float c = cos_f(phi), sn = sin_f(phi);
         const float ndx = dx * rinv, ndy = dy * rinv;
-        const float u = c * ndx + sn * ndy;
-        const float v = -sn * ndx + c * ndy;
+        /* the first product fused into the add, as nvcc contracts the
+         * reference's expressions by default (tile_forward.cu:259-260) */
+        const float u = fmaf(c, ndx, sn * ndy);
+        const float v = fmaf(-sn, ndx, c * ndy);
         const float ftx = (u + 1.f) * 0.5f * (float)(Wt - 1);
         const float fty = (v + 1.f) * 0.5f * (float)(Ht - 1);
         if (s->colors_orig && s->c_blend > 0.f)
