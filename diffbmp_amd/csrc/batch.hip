@@ -116,7 +116,8 @@ int ensure_prims(dbmp_batch* b, int N) {
   int rc;
   if ((rc = grow(&b->recs, cap))) return rc;
   if ((rc = grow(&b->counts, cap))) return rc;
-  if ((rc = grow(&b->offs, (long long)cap + 1))) return rc;
+  // offsets (cap + 1), then the scan's chunk sums
+  if ((rc = grow(&b->offs, (long long)cap + 1 + scan_partials_len(cap)))) return rc;
   if ((rc = grow(&b->ranges, cap))) return rc;
   if ((rc = grow(&b->grad8, 8LL * cap))) return rc;
   b->n_cap = cap;
@@ -183,8 +184,8 @@ int launch_step(dbmp_batch* b, hipStream_t st) {
                        prim_start(b), J, b->recs, b->counts, b->ranges);
     LAUNCH_CHECK("k_batch_prep");
   }
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, b->counts, N, b->offs, b->dtot);
-  LAUNCH_CHECK("k_scan_counts");
+  scan_counts(b->counts, N, b->offs, b->dtot, b->offs + b->n_cap + 1, st);
+  LAUNCH_CHECK("k_scan_apply");
   int bits = 0;
   while ((1 << bits) < b->tiles_tot) ++bits;
   const int passes = (bits + 7) / 8;
@@ -206,7 +207,8 @@ int launch_step(dbmp_batch* b, hipStream_t st) {
       hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, b->dtot, cap,
                          shift, nb, nblk, b->hist);
       LAUNCH_CHECK("k_radix_hist");
-      hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << nb), dim3(256), 0, st, b->hist, nblk, b->totals);
+      hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << nb), dim3(256), 0, st, b->hist, nblk, b->totals,
+                         b->dtot, cap);
       LAUNCH_CHECK("k_radix_rowscan");
       hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, vcur, kalt,
                          valt, b->dtot, cap, shift, nb, nblk, b->hist, b->totals);

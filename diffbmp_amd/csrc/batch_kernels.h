@@ -5,7 +5,7 @@
 // [tile_start[j], ...) and raster workgroups [block_start[j], ...).  One
 // launch of each stage covers all jobs:
 //   k_batch_prep    records + tile ranges, one thread per primitive
-//   k_scan_counts   (shared with the single-canvas path)
+//   k_scan_partials / k_scan_apply   (shared with the single-canvas path)
 //   k_batch_emit    pairs keyed by the job's tile base + local tile id, valued
 //                   by the local primitive id; the stable radix sort then
 //                   yields every job's lists, ascending primitive id per tile,

@@ -6,7 +6,8 @@
 //   k_prep        one thread per primitive: rotated template-extent AABB in
 //                 exact fp32 (every torch op rounded separately), closed-interval
 //                 tile range, pair count, and the raster constants (PrimRec).
-//   k_scan_counts exclusive 64-bit scan of the pair counts (one 1024-thread workgroup).
+//   k_scan_partials / k_scan_apply  exclusive 64-bit scan of the pair counts (one
+//                 1024-thread workgroup per chunk of 8192 counts).
 //   k_emit        one wave per primitive writes its (tile id, primitive id)
 //                 pairs, tiles in row-major order, primitives in id order.
 //   k_radix_*     stable LSD radix sort on the tile id; stability keeps the
@@ -249,44 +250,83 @@ __device__ __forceinline__ long long block_excl_scan(long long v, long long& tot
   return inc - v + before;
 }
 
-// offs[i] = sum(counts[0..i)), offs[N] = *total = sum(counts).  One
-// 1024-thread workgroup, 8 counts per thread per round (8192 per round).
+// offs[i] = sum(counts[0..i)), offs[N] = *total = sum(counts), over chunks
+// of kScanChunk counts, one 1024-thread workgroup per chunk (8 contiguous
+// counts per thread): k_scan_partials writes each chunk's sum, k_scan_apply
+// scans each chunk from the sum of the chunks before it.  With one chunk
+// only the second kernel runs.  No workgroup waits for another.
 constexpr int kScanThreads = 1024;
-static __global__ __launch_bounds__(kScanThreads) void k_scan_counts(const int* __restrict__ counts,
-                                                                     int N,
-                                                                     long long* __restrict__ offs,
-                                                                     long long* __restrict__ total) {
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__device__ __forceinline__ long long load_chunk_counts(const int* __restrict__ counts, int N, int i0,
+                                                       int loc[kScanItems]) {
+  if (i0 + kScanItems <= N) {
+    const int4 a = ld_i4(gbytes(counts + i0));
+    const int4 b = ld_i4(gbytes(counts + i0 + 4));
+    loc[0] = a.x; loc[1] = a.y; loc[2] = a.z; loc[3] = a.w;
+    loc[4] = b.x; loc[5] = b.y; loc[6] = b.z; loc[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) loc[j] = (i0 + j < N) ? gptr(counts)[i0 + j] : 0;
+  }
+  long long s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) s += loc[j];
+  return s;
+}
+
+static __global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int* __restrict__ counts,
+                                                                       int N,
+                                                                       long long* __restrict__ partials) {
   __shared__ long long sh[kScanThreads / 64];
-  constexpr int kItems = 8;
-  long long carry = 0;
-  for (int base = 0; base < N; base += kScanThreads * kItems) {
-    const int i0 = base + threadIdx.x * kItems;
-    int loc[kItems];
-    long long s = 0;
-    if (i0 + kItems <= N) {
-      const int4 a = *reinterpret_cast<const int4*>(counts + i0);
-      const int4 b = *reinterpret_cast<const int4*>(counts + i0 + 4);
-      loc[0] = a.x; loc[1] = a.y; loc[2] = a.z; loc[3] = a.w;
-      loc[4] = b.x; loc[5] = b.y; loc[6] = b.z; loc[7] = b.w;
-    } else {
+  int loc[kScanItems];
+  const long long s =
+      load_chunk_counts(counts, N, blockIdx.x * kScanChunk + threadIdx.x * kScanItems, loc);
+  long long tot;
+  (void)block_excl_scan<kScanThreads / 64>(s, tot, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+static __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* __restrict__ counts,
+                                                                    int N,
+                                                                    const long long* __restrict__ partials,
+                                                                    long long* __restrict__ offs,
+                                                                    long long* __restrict__ total) {
+  __shared__ long long sh[kScanThreads / 64];
+  // sum of the chunks before this one (at most a few hundred)
+  long long before = 0;
+  for (int i = threadIdx.x; i < (int)blockIdx.x; i += kScanThreads) before += partials[i];
+  long long pre;
+  (void)block_excl_scan<kScanThreads / 64>(before, pre, sh);
+  const int i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  int loc[kScanItems];
+  const long long s = load_chunk_counts(counts, N, i0, loc);
+  long long tot;
+  long long ex = block_excl_scan<kScanThreads / 64>(s, tot, sh) + pre;
 #pragma unroll
-      for (int j = 0; j < kItems; ++j) loc[j] = (i0 + j < N) ? counts[i0 + j] : 0;
-    }
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) s += loc[j];
-    long long tot;
-    long long ex = block_excl_scan<kScanThreads / 64>(s, tot, sh) + carry;
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      if (i0 + j < N) offs[i0 + j] = ex;
-      ex += loc[j];
-    }
-    carry += tot;
+  for (int j = 0; j < kScanItems; ++j) {
+    if (i0 + j < N) gptr(offs)[i0 + j] = ex;
+    ex += loc[j];
   }
-  if (threadIdx.x == 0) {
-    offs[N] = carry;
-    *total = carry;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    gptr(offs)[N] = pre + tot;
+    *gptr(total) = pre + tot;
   }
+}
+
+// Chunk sums needed by scan_counts for N counts (the buffer after offs).
+inline long long scan_partials_len(long long N) { return N / kScanChunk + 1; }
+
+// Queue the exclusive scan of counts[0..N) into offs[0..N] and *total;
+// partials holds scan_partials_len(N) values.
+inline void scan_counts(const int* counts, int N, long long* offs, long long* total,
+                        long long* partials, hipStream_t st) {
+  const int chunks = N > 0 ? (N + kScanChunk - 1) / kScanChunk : 1;
+  if (chunks > 1)
+    hipLaunchKernelGGL(k_scan_partials, dim3(chunks), dim3(kScanThreads), 0, st, counts, N, partials);
+  hipLaunchKernelGGL(k_scan_apply, dim3(chunks), dim3(kScanThreads), 0, st, counts, N,
+                     (const long long*)partials, offs, total);
 }
 
 // One wave per primitive; pairs land at offs[n] in row-major tile order.
@@ -336,18 +376,24 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(
   for (int d = threadIdx.x; d < radix; d += kRadixThreads) hist[(long)d * nblk + blockIdx.x] = h[d];
 }
 
-// One block per digit: exclusive scan of that digit's row over blocks.
+// One block per digit: exclusive scan of that digit's row over blocks.  The
+// rows are sized for the pair capacity; only the blocks that hold pairs are
+// scanned (the scatter reads no other, and the rest count nothing).
 static __global__ __launch_bounds__(256) void k_radix_rowscan(int* __restrict__ hist, int nblk,
-                                                       int* __restrict__ totals) {
+                                                       int* __restrict__ totals,
+                                                       const long long* __restrict__ total,
+                                                       long long cap) {
   __shared__ long long sh[4];
   int* row = hist + (long)blockIdx.x * nblk;
+  const long long P = pairs_in_range(total, cap);
+  const int nused = (int)min((long long)nblk, (P + kRadixTile - 1) / kRadixTile);
   long long carry = 0;
-  for (int base = 0; base < nblk; base += 256) {
+  for (int base = 0; base < nused; base += 256) {
     const int i = base + threadIdx.x;
-    const long long v = i < nblk ? row[i] : 0;
+    const long long v = i < nused ? row[i] : 0;
     long long tot;
     const long long ex = block256_excl_scan(v, tot, sh) + carry;
-    if (i < nblk) row[i] = (int)ex;
+    if (i < nused) row[i] = (int)ex;
     carry += tot;
   }
   if (threadIdx.x == 0) totals[blockIdx.x] = (int)carry;

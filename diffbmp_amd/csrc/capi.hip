@@ -142,7 +142,8 @@ int ensure_n(dbmp_rasterizer* h, int N) {
   int rc;
   if ((rc = grow(&h->recs, cap))) return rc;
   if ((rc = grow(&h->counts, cap))) return rc;
-  if ((rc = grow(&h->offs, (long long)cap + 1))) return rc;
+  // offsets (cap + 1), then the scan's chunk sums
+  if ((rc = grow(&h->offs, (long long)cap + 1 + scan_partials_len(cap)))) return rc;
   if ((rc = grow(&h->ranges, cap))) return rc;
   if ((rc = grow(&h->grad8, 8LL * cap))) return rc;
   h->n_cap = cap;
@@ -527,8 +528,8 @@ int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext
     LAUNCH_CHECK("k_tile_fill");
     return 0;
   }
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, h->counts, N, h->offs, dtot);
-  LAUNCH_CHECK("k_scan_counts");
+  scan_counts(h->counts, N, h->offs, dtot, h->offs + h->n_cap + 1, st);
+  LAUNCH_CHECK("k_scan_apply");
 
   int bits = 0;
   while ((1 << bits) < tiles) ++bits;
@@ -550,7 +551,8 @@ int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext
       hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, dtot,
                          (long long)capacity, sh, nb, nblk, h->hist);
       LAUNCH_CHECK("k_radix_hist");
-      hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << nb), dim3(256), 0, st, h->hist, nblk, h->totals);
+      hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << nb), dim3(256), 0, st, h->hist, nblk, h->totals,
+                         dtot, (long long)capacity);
       LAUNCH_CHECK("k_radix_rowscan");
       hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kRadixThreads), 0, st, kcur, vcur, kalt,
                          valt, dtot, (long long)capacity, sh, nb, nblk, h->hist, h->totals);
