@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--serve-max-jobs", type=int, default=32)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--serve-no-graph", action="store_true")
+    ap.add_argument("--serve-no-share", action="store_true",
+                    help="every job holds the texel records of its own bitmaps")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "reference kernel for scaling HBM byte counters)")
@@ -488,7 +490,8 @@ def run_serving(args, dev):
     warm = SV.JobSpec(256, 256, 512, (0,), steps=5, seed=1, name="warm")
     eng = SV.BatchEngine(store_dtype=torch.float16, use_graph=not args.serve_no_graph, device=dev)
     SV.Server(eng, SV.HbmBudget(1 << 34)).run([warm])
-    eng = SV.BatchEngine(store_dtype=torch.float16, use_graph=not args.serve_no_graph, device=dev)
+    eng = SV.BatchEngine(store_dtype=torch.float16, use_graph=not args.serve_no_graph, device=dev,
+                         share_bitmaps=not args.serve_no_share)
     budget = SV.HbmBudget(int(args.serve_budget_gib * (1 << 30)))
     srv = SV.Server(eng, budget, max_jobs=args.serve_max_jobs)
     stats = srv.run(specs)
@@ -511,7 +514,7 @@ def run_serving(args, dev):
                    "jobs": len(specs), "steps_per_job": args.serve_steps,
                    "arrival_rate": args.serve_rate or "all at t=0",
                    "max_jobs": args.serve_max_jobs, "hbm_budget_gib": args.serve_budget_gib,
-                   "graph": not args.serve_no_graph},
+                   "graph": not args.serve_no_graph, "shared_bitmaps": not args.serve_no_share},
         "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
         "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
         "queue_p50_s": round(float(np.percentile(q, 50)), 4),

@@ -169,6 +169,22 @@ def random_spec(rng: np.random.Generator, steps: int = 200, library_size: int = 
     return JobSpec(side, side, N, ids, steps=steps, seed=int(rng.integers(1 << 30)))
 
 
+def prepare_bitmaps(batch: "BatchEngine", templates, colors_orig, blend: bool):
+    """Extents and texel records (RGBA records with colour maps) of a template
+    set, in the engine's storage type: ``(ext, quads)``."""
+    dev, sdt = batch.device, batch.store_dtype
+    T, Ht, Wt = templates.shape
+    ext = torch.empty((T, 4), dtype=torch.float32, device=dev)
+    rec = _RGBA_BYTES[sdt] if blend else _QUAD_BYTES[sdt]
+    quads = torch.empty(T * Ht * Wt * rec, dtype=torch.uint8, device=dev)
+    bm = Bitmaps(templates.data_ptr(), colors_orig.data_ptr() if blend else None, T, Ht, Wt,
+                 _native._DT[sdt], 2 if blend else 0, 0.0, -1, 0, -1)
+    with torch.cuda.device(dev):
+        check(batch._L.dbmp_batch_prepare_bitmaps(batch._b, ctypes.byref(bm), _native._ptr(ext),
+                                                 _native._ptr(quads), _native._stream(dev)))
+    return ext, quads
+
+
 class FitState:
     """Device buffers of one admitted job."""
 
@@ -177,23 +193,21 @@ class FitState:
         self.spec = spec
         H, W, N = spec.height, spec.width, spec.num_primitives
         rng = np.random.default_rng(spec.seed)
-        lib_t, lib_c = device_library(spec.template_size, spec.library, sdt, dev)
-        ids = torch.tensor(list(spec.template_ids), dtype=torch.long, device=dev)
-        T = int(ids.numel())
-        Ht, Wt = lib_t.shape[1:]
-        self.templates = lib_t.index_select(0, ids)
-        self.colors_orig = lib_c.index_select(0, ids)
-        self.ext = torch.empty((T, 4), dtype=torch.float32, device=dev)
+        T = len(spec.template_ids)
         self.blend = spec.c_blend > 0
-        rec = _RGBA_BYTES[sdt] if self.blend else _QUAD_BYTES[sdt]
-        self.quads = torch.empty(T * Ht * Wt * rec, dtype=torch.uint8, device=dev)
-        bm = Bitmaps(self.templates.data_ptr(), self.colors_orig.data_ptr() if self.blend else None,
-                     T, Ht, Wt, _native._DT[sdt], 2 if self.blend else 0, float(spec.c_blend), -1, 0,
-                     -1)
-        with torch.cuda.device(dev):
-            check(batch._L.dbmp_batch_prepare_bitmaps(batch._b, ctypes.byref(bm),
-                                                     _native._ptr(self.ext), _native._ptr(self.quads),
-                                                     _native._stream(dev)))
+        if batch.share_bitmaps:
+            # the engine's one copy of the library's texel records; the job's
+            # primitives index the library directly
+            self.templates, self.colors_orig, self.ext, self.quads = batch.shared_bitmaps(
+                spec.template_size, spec.library, self.blend)
+            sel_map = np.asarray(spec.template_ids, np.int32)
+        else:
+            lib_t, lib_c = device_library(spec.template_size, spec.library, sdt, dev)
+            ids = torch.tensor(list(spec.template_ids), dtype=torch.long, device=dev)
+            self.templates = lib_t.index_select(0, ids)
+            self.colors_orig = lib_c.index_select(0, ids)
+            self.ext, self.quads = prepare_bitmaps(batch, self.templates, self.colors_orig, self.blend)
+            sel_map = np.arange(T, dtype=np.int32)
         lo, hi = spec.ls_range
         x = rng.uniform(0, W, N)
         y = rng.uniform(0, H, N)
@@ -207,7 +221,7 @@ class FitState:
         self.exp_avg = torch.zeros_like(self.params)
         self.exp_avg_sq = torch.zeros_like(self.params)
         self.store = self.params if sdt == torch.float32 else self.params.to(sdt)
-        self.sel = torch.from_numpy(((np.arange(N) % T)[::-1]).astype(np.int32).copy()).to(dev)
+        self.sel = torch.from_numpy(sel_map[(np.arange(N) % T)[::-1]].astype(np.int32)).to(dev)
         self.target = smooth_target(H, W, spec.seed + 1, dev)
         self.color = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
         self.alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
@@ -261,7 +275,11 @@ class BatchEngine:
 
     def __init__(self, store_dtype=torch.float16, betas=(0.9, 0.999), eps=1e-8,
                  use_graph: bool = True, pairs_per_primitive: float = 64.0, device=None,
-                 exact_blend: bool = False):
+                 exact_blend: bool = False, share_bitmaps: bool = True):
+        """``share_bitmaps``: jobs whose bitmaps come from the same library
+        share one copy of its texel records (the same samples from a
+        sixteenth or less of the distinct memory) instead of each holding
+        the records of its own subset."""
         self._L = _native.lib()
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
             else torch.device(device)
@@ -279,6 +297,17 @@ class BatchEngine:
         self.pair_capacity = 0
         self._array = None
         self.set_count = 0
+        self.share_bitmaps = bool(share_bitmaps)
+        self._shared: dict = {}
+
+    def shared_bitmaps(self, size: int, kind: str, blend: bool):
+        """The library ``(templates, colour maps, extents, records)`` on this
+        engine's device and storage type, built on first use."""
+        key = (size, kind, bool(blend))
+        if key not in self._shared:
+            tm, cm = device_library(size, kind, self.store_dtype, self.device)
+            self._shared[key] = (tm, cm, *prepare_bitmaps(self, tm, cm, blend))
+        return self._shared[key]
 
     def __del__(self):
         b = getattr(self, "_b", None)

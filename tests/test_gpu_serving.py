@@ -85,6 +85,31 @@ def test_job_alone_equals_in_mix(dev):
     assert l2[-1] < l2[0]
 
 
+def test_shared_bitmaps_equal_per_job_copies(dev):
+    """Jobs indexing the engine's one copy of the library's texel records
+    render what they render from their own copies: the same first images bit
+    for bit and the same trajectories within atomic-order noise (bitmap ids
+    out of order and not starting at 0)."""
+    specs = [SV.JobSpec(112, 96, 400, (5, 2, 11), steps=6, seed=21, name="S1", template_size=64),
+             SV.JobSpec(80, 144, 700, (15, 3), steps=6, seed=22, name="S2", template_size=64),
+             SV.JobSpec(64, 64, 150, (7,), steps=6, seed=23, name="S3", template_size=64)]
+    shared, own = engine(share_bitmaps=True), engine(share_bitmaps=False)
+    js, jo = states(shared, specs), states(own, specs)
+    assert js[0].templates.shape[0] == 16 and jo[0].templates.shape[0] == 3
+    shared.step()
+    own.step()
+    torch.cuda.synchronize()
+    for a, b in zip(js, jo):
+        assert torch.equal(a.color, b.color) and torch.equal(a.alpha, b.alpha), a.spec.name
+    for _ in range(5):
+        shared.step()
+        own.step()
+    torch.cuda.synchronize()
+    for a, b in zip(js, jo):
+        la, lb = a.loss.cpu().numpy(), b.loss.cpu().numpy()
+        assert np.allclose(la, lb, rtol=1e-4, atol=0), (a.spec.name, la, lb)
+
+
 def test_graph_replay_matches_eager(dev):
     eg, ee = engine(graph=True), engine(graph=False)
     sg, se = states(eg, [A, B]), states(ee, [A, B])
