@@ -63,7 +63,8 @@ class Bitmaps(ctypes.Structure):
 class TileLists(ctypes.Structure):
     _fields_ = [("tile_offsets", ctypes.c_void_p), ("tile_indices", ctypes.c_void_p),
                 ("pair_count", ctypes.c_void_p), ("capacity", ctypes.c_int64),
-                ("bin_factor", ctypes.c_int32), ("ranges", ctypes.c_void_p)]
+                ("bin_factor", ctypes.c_int32), ("ranges", ctypes.c_void_p),
+                ("push_masks", ctypes.c_void_p), ("push_mask_words", ctypes.c_int64)]
 
 
 class Job(ctypes.Structure):
@@ -97,20 +98,25 @@ class Lists:
     """Tile lists on the device: CSR offsets, index buffer, and -- for lists
     produced by dbmp_bin -- the device pair count and the capacity used.
     ``factor`` > 1 marks super-tile lists from dbmp_bin_tiles (one list per
-    factor x factor tiles) with the binning's per-primitive tile ``ranges``."""
+    factor x factor tiles) with the binning's per-primitive tile ``ranges``.
+    ``masks`` (int64 words, optional) is forward->backward state: the forward
+    over these lists records which entries reached each wave's pixels and the
+    backward of that forward walks only those."""
 
-    __slots__ = ("offsets", "indices", "count", "capacity", "factor", "ranges")
+    __slots__ = ("offsets", "indices", "count", "capacity", "factor", "ranges", "masks")
 
-    def __init__(self, offsets, indices, count=None, capacity=0, factor=1, ranges=None):
+    def __init__(self, offsets, indices, count=None, capacity=0, factor=1, ranges=None, masks=None):
         self.offsets, self.indices, self.count, self.capacity = offsets, indices, count, int(capacity)
-        self.factor, self.ranges = int(factor), ranges
+        self.factor, self.ranges, self.masks = int(factor), ranges, masks
 
     def c(self) -> TileLists:
         idx = self.indices if self.indices.numel() else self.offsets
+        m = self.masks
         return TileLists(self.offsets.data_ptr(), idx.data_ptr(),
                          self.count.data_ptr() if self.count is not None else None,
                          self.capacity, self.factor,
-                         self.ranges.data_ptr() if self.ranges is not None else None)
+                         self.ranges.data_ptr() if self.ranges is not None else None,
+                         m.data_ptr() if m is not None else None, m.numel() if m is not None else 0)
 
 
 EXPORTS = {
@@ -120,6 +126,7 @@ EXPORTS = {
                                  ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                  ctypes.c_int]),
     "dbmp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "dbmp_push_mask_words": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32]),
     "dbmp_template_extents": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Bitmaps),
                                              ctypes.c_void_p, ctypes.c_void_p]),
     "dbmp_bin": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Prims), ctypes.c_void_p,
@@ -451,14 +458,20 @@ class Handle:
             self.pair_capacity = int(out[1].numel() * 1.3) + 4096
             return Lists(out[0], out[1], factor=factor, ranges=out[2] if factor > 1 else None)
         cap = self.pair_capacity
-        # one allocation: count (int64) | pad | ranges (N,4) | offsets | indices
+        # one allocation: count (int64) | pad | ranges (N,4) | offsets | pad |
+        # push masks (int64 words) | indices
         L = self.list_count(factor) + 1
         nr = 4 * max(N, 1)
-        buf = torch.empty(4 + nr + ((L + 3) & ~3) + cap, dtype=torch.int32, device=dev)
+        nw = int(self._L.dbmp_push_mask_words(self._h, cap, int(factor)))
+        o_offs = 4 + nr
+        o_pm = o_offs + ((L + 3) & ~3)
+        o_idx = o_pm + 2 * nw
+        buf = torch.empty(o_idx + cap, dtype=torch.int32, device=dev)
         cnt = buf[0:2].view(torch.int64)
-        rg = buf[4:4 + nr].view(-1, 4)
-        offs = buf[4 + nr:4 + nr + L]
-        idx = buf[4 + nr + ((L + 3) & ~3):]
+        rg = buf[4:o_offs].view(-1, 4)
+        offs = buf[o_offs:o_offs + L]
+        pm = buf[o_pm:o_idx].view(torch.int64)
+        idx = buf[o_idx:]
         self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
         if self._pending is None and self._lazy_calls % self.READBACK_EVERY == 0:
             if self._readback is None:
@@ -467,7 +480,7 @@ class Handle:
             self._readback.copy_(cnt, non_blocking=True)
             self._readback_ev.record(torch.cuda.current_stream(dev))
             self._pending = self._readback_ev
-        return Lists(offs, idx, cnt, cap, factor, rg)
+        return Lists(offs, idx, cnt, cap, factor, rg, pm)
 
     def forward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T: float, out_color, out_alpha,
                 out_T, out_last) -> None:

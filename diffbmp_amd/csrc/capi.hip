@@ -127,6 +127,7 @@ struct dbmp_rasterizer {
   // per-template content boxes of the templates the quads hold (culling)
   int cull_cap = 0;
   float4* cull = nullptr;
+
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_fwd, ev_bwd;
@@ -308,6 +309,13 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   return 0;
 }
 
+// Push-mask words for lists of at most `capacity` entries in all, `lists`
+// lists and `subs` 16x16 blocks per list tile: pm_word stays below
+// floor(capacity / 64) + lists per (block of a list tile, wave).
+long long push_mask_words(long long capacity, int lists, int subs) {
+  return ((capacity >> 6) + lists + 1) * (long long)subs * 4;
+}
+
 RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps* b,
                      const dbmp_tile_lists* L, float eps) {
   RasterArgs A;
@@ -359,6 +367,11 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
     A.tiles = A.tilesX * ((h->tilesY + f - 1) / f);
     A.subs = f * f;
   }
+  // push masks: lists with a pair count and capacity bound the mask words
+  A.pmask = nullptr;
+  if (L->push_masks != nullptr && L->pair_count != nullptr &&
+      L->push_mask_words >= push_mask_words(L->capacity, A.tiles, A.subs))
+    A.pmask = (unsigned long long*)L->push_masks;
   return A;
 }
 
@@ -409,6 +422,13 @@ void timing_drain(std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, double& ms,
 extern "C" {
 
 int dbmp_version(void) { return 100; }
+
+int64_t dbmp_push_mask_words(const dbmp_rasterizer* h, int64_t capacity, int32_t bin_factor) {
+  if (!h || capacity < 0) return 0;
+  const int f = bin_factor > 1 ? bin_factor : 1;
+  const int lists = ((h->tilesX + f - 1) / f) * ((h->tilesY + f - 1) / f);
+  return push_mask_words(capacity, lists, h->subs * f * f);
+}
 
 const char* dbmp_last_error(void) { return g_err.c_str(); }
 
@@ -781,6 +801,7 @@ int dbmp_forward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
   RasterArgs A = make_args(h, &prims[0], bm, &lists[0], eps_T);
   A.order = nullptr;
   A.cost = nullptr;
+  A.pmask = nullptr;   // push masks are per canvas; candidates cull
   const int nb = A.tiles * A.subs;
   const long long px = (long long)h->H * h->W;
   const bool blend = A.cmap != nullptr;
@@ -831,6 +852,7 @@ int dbmp_backward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
   RasterArgs A = make_args(h, &prims[0], bm, &lists[0], 0.f);
   A.order = nullptr;
   A.cost = nullptr;
+  A.pmask = nullptr;   // push masks are per canvas; candidates cull
   const int nb = A.tiles * A.subs;
   const long long px = (long long)h->H * h->W;
   const bool blend = A.cmap != nullptr;

@@ -62,6 +62,13 @@ struct RasterArgs {
   int* cost;             // forward: each block's walk cost, or null
   const float4* cull;    // per template: content box (k_template_extents<_, true>), or null
   int ntmpl;             // templates in `cull`
+  // Push masks, or null: per (list batch, block, wave) one 64-bit word whose
+  // bit j is set when entry j of the batch pushed a lane of the wave.  The
+  // forward writes them; a backward of that forward (same lists, parameters
+  // and per-pixel state) walks exactly those entries instead of culling.
+  // Word of batch i of list t: pm_word below.  Unused after a binning
+  // overflow (every primitive walked).
+  unsigned long long* pmask;
 };
 
 // Block a workgroup runs: the heaviest-first order of the last forward's
@@ -183,7 +190,7 @@ constexpr int kBatch = 64;    // list entries staged per batch
 constexpr int kThreads = 256; // 4 waves per 16x16 block
 
 struct BlockGeom {
-  int tile, bx, by, px, py, w, lane;
+  int tile, sub, bx, by, px, py, w, lane;
   int rtx, rty;   // the block's tile in the units of A.ranges
 };
 
@@ -191,11 +198,11 @@ struct BlockGeom {
 __device__ __forceinline__ BlockGeom block_geom(const RasterArgs& A, int b) {
   BlockGeom g;
   g.tile = b / A.subs;
-  const int sub = b - g.tile * A.subs;
+  g.sub = b - g.tile * A.subs;
   const int spr = A.ts / kSub;
   const int tx = g.tile % A.tilesX, ty = g.tile / A.tilesX;
-  g.bx = tx * A.ts + (sub % spr) * kSub;
-  g.by = ty * A.ts + (sub / spr) * kSub;
+  g.bx = tx * A.ts + (g.sub % spr) * kSub;
+  g.by = ty * A.ts + (g.sub / spr) * kSub;
   // ranges count 16-px tiles for super-tile lists, list tiles otherwise
   g.rtx = A.bin_shift > 0 ? g.bx >> 4 : tx;
   g.rty = A.bin_shift > 0 ? g.by >> 4 : ty;
@@ -219,6 +226,16 @@ __device__ __forceinline__ bool list_range(const RasterArgs& A, int tile, int& s
   start = gptr(A.toffs)[tile];
   end = gptr(A.toffs)[tile + 1];
   return false;
+}
+
+// Push-mask word of batch i of the block's list (list t = G.tile starting at
+// entry `start`) for wave G.w.  List t's batches take words
+// floor(start_t / 64) + t + i, i < ceil(len_t / 64); the next list starts at
+// floor((start_t + len_t) / 64) + t + 1 or later, so the words of different
+// lists never meet, and every word is below floor(P / 64) + lists for P list
+// entries in all.  Each word holds the block's sub-blocks and waves.
+__device__ __forceinline__ long pm_word(const RasterArgs& A, const BlockGeom& G, int start, int i) {
+  return ((long)((start >> 6) + G.tile + i) * A.subs + G.sub) * 4 + G.w;
 }
 
 // Load one 16-byte quarter of the record of list entry k (thread t handles
@@ -284,6 +301,16 @@ __device__ __forceinline__ bool entry_hits_wave(const RasterArgs& A, const PrimR
   bool in = (smask[lane] & wbit) != 0;
   if (A.cull != nullptr) in = in & content_hits(A, srec[lane], cx, cy);
   return in;
+}
+
+// Clear / set bit j of a wave-uniform 64-bit mask in one scalar instruction
+// (wm & (wm - 1) takes three).  The raster loops are bound by the total
+// instruction count per list entry, scalar instructions included.
+__device__ __forceinline__ void sbit_clear(unsigned long long& m, int j) {
+  asm("s_bitset0_b64 %0, %1" : "+s"(m) : "s"(j));
+}
+__device__ __forceinline__ void sbit_set(unsigned long long& m, int j) {
+  asm("s_bitset1_b64 %0, %1" : "+s"(m) : "s"(j));
 }
 
 // Lane-mask vote on a bool: the ballot stays a scalar mask (no per-lane 0/1
@@ -401,8 +428,11 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     if (A.cost != nullptr && threadIdx.x == 0) gptr(A.cost)[b] = 0;
     return;
   }
-  const float fpx = (float)G.px, fpy = (float)G.py;
-  bool alive = (G.px < A.W) && (G.py < A.H);
+  const bool inside = (G.px < A.W) && (G.py < A.H);
+  // A lane that is not compositing (past the canvas edge, or stopped) has a
+  // NaN pixel coordinate: its push-box test is false, so it pushes nothing,
+  // and its liveness is `thr >= 0` -- no lane mask to carry per entry.
+  float fpx = inside ? (float)G.px : __builtin_nanf(""), fpy = (float)G.py;
   // Tnz: product of the nonzero (1 - a) factors; nz: count of zero factors
   // (exactly opaque layers), capped at 2.  Until a lane meets an exactly
   // opaque layer Tnz is its transmittance T; from then on T is 0, the lane's
@@ -410,13 +440,16 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   // accumulating are dropped.  One running product instead of two.
   float Tnz = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, Cf0 = 0.f, Cf1 = 0.f, Cf2 = 0.f;
   int nz = 0;
-  float thr = alive ? A.eps : -1.f;   // stop test threshold (below)
+  // stop test threshold: eps_T on a live lane, +inf once it has seen two
+  // opaque layers, -1 on a stopped lane (or one past the canvas edge)
+  float thr = inside ? A.eps : -1.f;
   int last = -1;
   int walked = 0;   // entries this wave evaluated (wave-uniform)
   LaneTally tally;
   int batches = 0;
   int start, end;
   const bool all = list_range(A, G.tile, start, end);
+  DBMP_GLOBAL unsigned long long* const pm = (A.pmask != nullptr && !all) ? gptr(A.pmask) : nullptr;
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
   const int wbit = 1 << G.w;
   const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
@@ -429,23 +462,25 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     ++batches;
     // entries of this batch that can touch this wave's pixels (before the
     // next batch's loads are in flight: fewer live registers)
-    const bool any_alive = wave_any(alive);
+    const bool any_alive = wave_any(thr >= 0.f);
     unsigned long long wm = 0ull;
     if (any_alive)
       wm = wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wbit, ccx, ccy));
     Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
+    unsigned long long hits = 0ull;   // entries of this batch that pushed a lane of the wave
     if (any_alive) {
+      walked += __popcll(wm);   // less the entries left when the wave stops (below)
       while (wm) {
         const int j = __builtin_ctzll(wm);
-        wm &= wm - 1ull;
-        ++walked;
+        sbit_clear(wm, j);
         const PrimRec p = srec[buf][j];
-        const Sample<DT> S = eval_sample<DT, false, BLEND>(A, p, fpx, fpy, alive);
+        const Sample<DT> S = eval_sample<DT, false, BLEND>(A, p, fpx, fpy, true);
         tally.add(S.pushed);
         if constexpr (BLEND) {
           // skip the colour-map gathers when nothing of this entry reaches the wave
           if (!wave_any(S.pushed)) continue;
+          sbit_set(hits, j);
         }
         float col[3];
         if (BLEND && A.cmap != nullptr) {
@@ -456,6 +491,12 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
           col[0] = p.cr; col[1] = p.cg; col[2] = p.cb;
         }
         const float a = S.pushed ? p.abase * S.m : 0.f;   // a = 0: compositing is a no-op
+        if constexpr (!BLEND) {
+          // a > 0 on every pushed lane except where alpha_ub sigma(v) mask is 0,
+          // whose gradients are 0 too (a ballot of `pushed` itself costs two
+          // VALU instructions here: the compiler materialises the predicate)
+          if (wave_any(a > 0.f)) sbit_set(hits, j);
+        }
         const float wgt = Tnz * a;
         C0 = fmaf(wgt, col[0], C0);
         C1 = fmaf(wgt, col[1], C1);
@@ -464,9 +505,8 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         const float f = fminf(fmaxf(1.f - a, 0.f), 1.f);
         // a pixel stops once later layers can change neither its colour nor
         // its alpha (or alpha's derivative w.r.t. an opaque layer) by eps_T
-        // (Tnz < thr: thr is eps_T on a live lane, +inf once it has seen two
-        // opaque layers, -1 on a stopped lane; a live lane that is not pushed
-        // keeps Tnz >= eps_T, since eps_T < 1, so the test needs no `pushed`)
+        // (Tnz < thr; a live lane that is not pushed keeps Tnz >= eps_T,
+        // since eps_T < 1, so the test needs no `pushed`)
         // nz counts exactly opaque layers (capped at 2), rare enough for a
         // wave-uniform branch
         const bool opq = f == 0.f;
@@ -478,22 +518,26 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
           Cf1 = first ? C1 : Cf1;
           Cf2 = first ? C2 : Cf2;
           nz = opq ? min(nz + 1, 2) : nz;
-          thr = (alive & (nz >= 2)) ? __builtin_inff() : thr;
+          thr = ((thr >= 0.f) & (nz >= 2)) ? __builtin_inff() : thr;
         }
         const bool dead = Tnz < thr;
         if (wave_any(dead)) {
           last = dead ? base - start + j + 1 : last;
-          alive = alive & !dead;
           thr = dead ? -1.f : thr;
-          if (!wave_any(alive)) break;
+          fpx = dead ? __builtin_nanf("") : fpx;
+          if (!wave_any(thr >= 0.f)) {   // the wave has stopped: leave the batch
+            walked -= __popcll(wm);
+            wm = 0ull;
+          }
         }
       }
     }
+    if (pm != nullptr && G.lane == 0) pm[pm_word(A, G, start, (base - start) >> 6)] = hits;
     if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G);
     // one barrier: it publishes the staged batch and each wave's liveness
     // (one flag per wave and buffer parity, so a flag is rewritten only after
     // every wave has read it; __syncthreads_or takes three barriers)
-    const int wave_alive = wave_any(alive) ? 1 : 0;
+    const int wave_alive = wave_any(thr >= 0.f) ? 1 : 0;
     if (G.lane == 0) salive[buf][G.w] = wave_alive;
     __syncthreads();
     const int4 fl = *reinterpret_cast<const int4*>(salive[buf]);
@@ -610,6 +654,11 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
   const float ccy = uniform_f((float)(G.by + 8 * (G.w >> 1)) + 3.5f);
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G);
+  // the forward's push masks (one word per batch, wave-uniform), loaded a
+  // batch ahead
+  const DBMP_GLOBAL unsigned long long* const pm =
+      (A.pmask != nullptr && !all) ? gptr((const unsigned long long*)A.pmask) : nullptr;
+  unsigned long long pm_cur = pm != nullptr ? pm[pm_word(A, G, start, 0)] : 0ull;
   __syncthreads();
   const int stop = start + min(skmax, end - start);
   LaneTally tally;
@@ -617,18 +666,26 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   for (int base = start; base < stop; base += kBatch, buf ^= 1) {
     const bool have_next = base + kBatch < stop;
     const int cnt = __builtin_amdgcn_readfirstlane(min(kBatch, stop - base));
-    // entries of this batch that can touch this wave's pixels, up to the
-    // wave's furthest forward position (before the next batch's loads are
-    // in flight: fewer live registers)
-    unsigned long long wm =
-        wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wbit, ccx, ccy));
+    // entries of this batch that pushed a lane of this wave in the forward,
+    // or (no masks) that can touch this wave's pixels, up to the wave's
+    // furthest forward position (before the next batch's loads are in
+    // flight: fewer live registers)
+    unsigned long long wm;
+    if (pm != nullptr)
+      wm = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(pm_cur >> 32)) << 32) |
+           (unsigned)__builtin_amdgcn_readfirstlane((unsigned)pm_cur);
+    else
+      wm = wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wbit, ccx, ccy));
     const int lim = wkmax - (base - start);
     wm = lim >= 64 ? wm : (lim <= 0 ? 0ull : wm & ((1ull << lim) - 1ull));
     Stage nxt;
-    if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
+    if (have_next) {
+      nxt = stage_load(A, all, base + kBatch + ri, end, part);
+      if (pm != nullptr) pm_cur = pm[pm_word(A, G, start, ((base - start) >> 6) + 1)];
+    }
     while (wm) {
       const int j = __builtin_ctzll(wm);
-      wm &= wm - 1ull;
+      sbit_clear(wm, j);
       const int kk = base - start + j;
       const PrimRec p = srec[buf][j];
       const Sample<DT> S = eval_sample<DT, true, BLEND>(A, p, fpx, fpy, kk < lst);
@@ -687,7 +744,11 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       // (:314-343); both zero on lanes not pushed
       const float dLdap = S.pushed ? dLda : 0.f;
       acc[4] = dLdap * S.m;
-      const float dLdm = dLdap * p.abase;
+      // dL/dmask = dL/da alpha_ub sigma(v).  Without colour maps the factor
+      // alpha_ub sigma(v) is constant per primitive and applied once per
+      // entry below; the exact colour-map term added to dL/du, dL/dv is not
+      // proportional to it, so the blending kernels apply it per lane
+      const float dLdm = BLEND ? dLdap * p.abase : dLdap;
       // dL/du and dL/dv divided by hsx = dtx/du (hsx is applied with the
       // per-primitive factors); hyx = hsy / hsx
       float dLdu = dLdm * S.gxt;
@@ -700,15 +761,17 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
           dLdv = fmaf(kw * hyx, fmaf(gc0, dcy[0], fmaf(gc1, dcy[1], gc2 * dcy[2])), dLdv);
         }
       }
-      // (u, v) = R(theta) (dx, dy) / r: with (gx, gy) = R^T (dL/du, dL/dv),
+      // (u, v) = R(theta) (dx, dy) / r.  With (gx, gy) = R^T (dL/du, dL/dv):
       //   dL/dmx = -gx/r, dL/dmy = -gy/r, dL/dr = -(ndx gx + ndy gy)/r,
-      //   dL/dtheta = ndy gx - ndx gy
-      const float gx = fmaf(p.c, dLdu, -p.s * dLdv);
-      const float gy = fmaf(p.s, dLdu, p.c * dLdv);
-      acc[0] = gx;
-      acc[1] = gy;
-      acc[2] = fmaf(S.ndx, gx, S.ndy * gy);
-      acc[3] = fmaf(S.ndy, gx, -S.ndx * gy);
+      //   dL/dtheta = ndy gx - ndx gy,
+      // where, R being a rotation, ndx gx + ndy gy = u dL/du + v dL/dv and
+      // ndy gx - ndx gy = v dL/du - u dL/dv.  The lanes sum dL/du, dL/dv and
+      // those two; R^T (constant per primitive) is applied to the sums once
+      // per entry below
+      acc[0] = dLdu;
+      acc[1] = dLdv;
+      acc[2] = fmaf(S.u, dLdu, S.v * dLdv);
+      acc[3] = fmaf(S.v, dLdu, -S.u * dLdv);
       T = Tn;
       int which;
       const float tot = wave_reduce_scatter8(acc, which);
@@ -723,9 +786,15 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const int j = i >> 3, q = i & 7;
       const PrimRec& pr = srec[buf][j];
       const float cc = q == 5 ? pr.cr : (q == 6 ? pr.cg : pr.cb);
-      const float fac = q < 3 ? -pr.rinv * A.hsx
-                              : (q == 3 ? A.hsx : (q == 4 ? pr.dav : cc * (1.f - cc) * cwf));
-      const float s = (sred[0][j][q] + sred[1][j][q] + sred[2][j][q] + sred[3][j][q]) * fac;
+      const float fa = BLEND ? 1.f : pr.abase;   // alpha_ub sigma(v) of the mask path
+      const float fac = q < 3 ? -pr.rinv * A.hsx * fa
+                              : (q == 3 ? A.hsx * fa : (q == 4 ? pr.dav : cc * (1.f - cc) * cwf));
+      float s = sred[0][j][q] + sred[1][j][q] + sred[2][j][q] + sred[3][j][q];
+      // (gx, gy) = R^T (sum dL/du, sum dL/dv): values 0 and 1 of an entry sit
+      // in neighbouring lanes, active together (cnt * 8 items in all)
+      const float so = __shfl_xor(s, 1, 64);
+      s = q == 0 ? fmaf(pr.c, s, -pr.s * so) : (q == 1 ? fmaf(pr.s, so, pr.c * s) : s);
+      s *= fac;
       sred[0][j][q] = 0.f;
       sred[1][j][q] = 0.f;
       sred[2][j][q] = 0.f;

@@ -98,7 +98,23 @@ typedef struct {
                                lists (dbmp_bin_tiles' ranges_out); required for f > 1.
                                For f <= 1 with pair_count set, NULL means the ranges
                                the handle's latest dbmp_bin wrote */
+  uint64_t* push_masks;     /* optional forward->backward state, like out_last: for
+                               each 64-entry batch of a list, 16x16 block and wave,
+                               which entries pushed a pixel of the wave.  dbmp_forward
+                               writes them and a backward of that forward (same lists,
+                               parameters and out_last) walks only those entries
+                               instead of culling; results are the same.  Used when
+                               pair_count is set and push_mask_words is at least
+                               dbmp_push_mask_words(h, capacity, bin_factor); NULL:
+                               none.  Not used by the candidate (batch) calls */
+  int64_t push_mask_words;  /* uint64 words at push_masks */
 } dbmp_tile_lists;
+
+/* Words of dbmp_tile_lists.push_masks for lists of at most `capacity` entries
+ * with this bin_factor on this handle's canvas.  No reference counterpart
+ * (the reference re-tests every list entry in its backward,
+ * tile_backward.cu:127-157). */
+int64_t dbmp_push_mask_words(const dbmp_rasterizer* h, int64_t capacity, int32_t bin_factor);
 
 /* Library version (major*10000 + minor*100 + patch). */
 int dbmp_version(void);
