@@ -590,20 +590,35 @@ struct LossArgs {
 // Backward of one 16x16 block b.  With LOSS the pixel gradients come from the
 // fused background composite + MSE (L; gCin / gAin unused), else from gCin /
 // gAin (L unused).
-template <int DT, bool BLEND, bool LOSS>
-__device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
-                                                 const float* __restrict__ gCin,
-                                                 const float* __restrict__ gAin,
-                                                 const LossArgs& L,
-                                                 const float* __restrict__ Cfin,
-                                                 const float* __restrict__ Tfin,
-                                                 const int* __restrict__ lastin,
-                                                 float* __restrict__ grad8) {
-  __shared__ PrimRec srec[2][kBatch];
-  __shared__ int sid[2][kBatch];
-  __shared__ int smask[2][kBatch];
-  __shared__ float sred[4][kBatch][8];
-  __shared__ int skmax;
+// The backward's LDS: staged records (double buffered), the four waves'
+// gradient partials of a batch, the block's furthest forward position.
+struct BwdLds {
+  PrimRec srec[2][kBatch];
+  int sid[2][kBatch];
+  int smask[2][kBatch];
+  float sred[4][kBatch][8];
+  int skmax;
+};
+
+// MASKS: compiled for lists that carry push masks (A.pmask): each batch's
+// entries come from the forward's mask, every one of them pushed a lane of
+// the wave, so there is no cull and no per-entry push test (after a binning
+// overflow, which writes no masks, the entries are culled).  Without MASKS
+// the batch is culled and entries that push no lane are skipped.
+template <int DT, bool BLEND, bool LOSS, bool MASKS>
+__device__ __forceinline__ void raster_bwd_body(BwdLds& lds, const RasterArgs& A, int b,
+                                                const float* __restrict__ gCin,
+                                                const float* __restrict__ gAin,
+                                                const LossArgs& L,
+                                                const float* __restrict__ Cfin,
+                                                const float* __restrict__ Tfin,
+                                                const int* __restrict__ lastin,
+                                                float* __restrict__ grad8) {
+  PrimRec (*srec)[kBatch] = lds.srec;
+  int (*sid)[kBatch] = lds.sid;
+  int (*smask)[kBatch] = lds.smask;
+  float (*sred)[kBatch][8] = lds.sred;
+  int& skmax = lds.skmax;
   const BlockGeom G = block_geom(A, b);
   if (G.bx >= A.W || G.by >= A.H) return;   // 16-px block of a tile past the canvas edge
   const float fpx = (float)G.px, fpy = (float)G.py;
@@ -657,7 +672,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
   // the forward's push masks (one word per batch, wave-uniform), loaded a
   // batch ahead
   const DBMP_GLOBAL unsigned long long* const pm =
-      (A.pmask != nullptr && !all) ? gptr((const unsigned long long*)A.pmask) : nullptr;
+      (MASKS && A.pmask != nullptr && !all) ? gptr((const unsigned long long*)A.pmask) : nullptr;
   unsigned long long pm_cur = pm != nullptr ? pm[pm_word(A, G, start, 0)] : 0ull;
   __syncthreads();
   const int stop = start + min(skmax, end - start);
@@ -690,7 +705,10 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       const PrimRec p = srec[buf][j];
       const Sample<DT> S = eval_sample<DT, true, BLEND>(A, p, fpx, fpy, kk < lst);
       tally.add(S.pushed);
-      if (!wave_any(S.pushed)) continue;
+      // every entry of a push mask pushed a lane; without masks skip the
+      // culled entries that push none (an entry with no pushed lane adds 0
+      // to every gradient either way)
+      if (!MASKS && !wave_any(S.pushed)) continue;
       float col[3], dcx[3] = {0.f, 0.f, 0.f}, dcy[3] = {0.f, 0.f, 0.f};
       const bool has_map = BLEND && A.cmap != nullptr;
       if (has_map) {
@@ -721,13 +739,12 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       // the general form: for an exactly opaque layer B_k is the
       // nonzero-factor product after it, tnzf / T_k, if it is the only opaque
       // layer, else 0
-      float TkB;
+      float TkB = tf * inv1m;
       if (wave_any(fk < 1e-6f)) {
         const bool tpos = Tn > 0.f;
         const float rB = fminf((tpos ? tf : tnzf) * __builtin_amdgcn_rcpf(tpos ? Tn : Tk), 1.f);
-        TkB = Tk * (tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f));
-      } else {
-        TkB = tf * inv1m;
+        const float g = Tk * (tpos ? rB : (fk == 0.f ? (nzf == 1 ? rB : 0.f) : 1.f));
+        TkB = fk < 1e-6f ? g : TkB;
       }
       const float dLda = fmaf(ga, TkB, fmaf(Tk, gcol, -gS * inv1m));
       // Per-sample parts of the 8 gradients.  Factors that are constant per
@@ -775,7 +792,10 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       T = Tn;
       int which;
       const float tot = wave_reduce_scatter8(acc, which);
-      if ((G.lane & 7) == 0) sred[G.w][j][which] = tot;
+      // the eight lanes of a group hold the same total: all of them write it
+      // (masking the write to one lane costs more instructions than the
+      // repeated LDS stores)
+      sred[G.w][j][which] = tot;
     }
     if (have_next) stage_store(nxt, srec, sid, smask, buf ^ 1, ri, part, G);
     __syncthreads();
@@ -817,6 +837,22 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
       atomic_add_global(L.loss, (sred[0][0][0] + sred[0][0][1] + sred[0][0][2] + sred[0][0][3]) *
                                   L.inv_count);
   }
+}
+
+template <int DT, bool BLEND, bool LOSS>
+__device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
+                                                 const float* __restrict__ gCin,
+                                                 const float* __restrict__ gAin,
+                                                 const LossArgs& L,
+                                                 const float* __restrict__ Cfin,
+                                                 const float* __restrict__ Tfin,
+                                                 const int* __restrict__ lastin,
+                                                 float* __restrict__ grad8) {
+  __shared__ BwdLds lds;
+  if (A.pmask != nullptr)
+    raster_bwd_body<DT, BLEND, LOSS, true>(lds, A, b, gCin, gAin, L, Cfin, Tfin, lastin, grad8);
+  else
+    raster_bwd_body<DT, BLEND, LOSS, false>(lds, A, b, gCin, gAin, L, Cfin, Tfin, lastin, grad8);
 }
 
 // Backward with the background composite + MSE as its per-pixel prologue.
