@@ -254,23 +254,25 @@ __device__ __forceinline__ float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// Bit w set when the push box (quarter 0 of the record: mx, my, hw, hh) can
-// touch wave w's pixels [x0, x0+7] x [y0, y0+7], x0 = bx + 8 (w & 1),
-// y0 = by + 8 (w >> 1).  Conservative w.r.t. the per-pixel fp32 test: the
-// box is widened to whole pixels (floor of its low edges, floor + 1 of its
-// high edges) and compared in integers against the block's scalar bounds
-// (float-to-int conversion saturates; no +1 on a converted value, so no
-// overflow).
-__device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
-  const float mx = q.x, my = q.y, hw = q.z, hh = q.w;
-  if (!(hw >= 0.f && hh >= 0.f)) return 0;
-  const int x0 = (int)floorf(mx - hw), x1 = (int)floorf(mx + hw);
-  const int y0 = (int)floorf(my - hh), y1 = (int)floorf(my + hh);
-  const bool xl = (x0 <= bx + 7) & (x1 >= bx - 1);
-  const bool xr = (x0 <= bx + 15) & (x1 >= bx + 7);
-  const bool yt = (y0 <= by + 7) & (y1 >= by - 1);
-  const bool yb = (y0 <= by + 15) & (y1 >= by + 7);
-  return (int)(xl & yt) | ((int)(xr & yt) << 1) | ((int)(xl & yb) << 2) | ((int)(xr & yb) << 3);
+// The wave's pixel square [x0, x0+7] x [y0, y0+7] widened by one pixel and
+// to the pixel edges, as the bounds a push box must reach: from the centre
+// (cx, cy) = (x0 + 3.5, y0 + 3.5), lo = x0 - 1 and hi = x0 + 8 (exact floats).
+struct WaveBox {
+  float xlo, xhi, ylo, yhi;
+};
+__device__ __forceinline__ WaveBox wave_box(float cx, float cy) {
+  return WaveBox{uniform_f(cx - 4.5f), uniform_f(cx + 4.5f), uniform_f(cy - 4.5f),
+                 uniform_f(cy + 4.5f)};
+}
+
+// Whether the push box (quarter 0 of the record: mx, my, hw, hh) can touch
+// the wave's pixels.  Conservative w.r.t. the per-pixel fp32 test
+// |px - mx| <= hw: the box widened to whole pixels, floor(mx - hw) <= x0 + 7
+// and floor(mx + hw) >= x0 - 1, i.e. mx - hw < x0 + 8 and mx + hw >= x0 - 1
+// (the same decisions, without float-to-int conversions).  NaN rejects.
+__device__ __forceinline__ bool box_hits(const PrimRec& p, const WaveBox& b) {
+  return (p.mx - p.hw < b.xhi) & (p.mx + p.hw >= b.xlo) & (p.my - p.hh < b.yhi) &
+         (p.my + p.hh >= b.ylo);
 }
 
 // Whether the content of a staged entry's template can reach the wave's
@@ -278,7 +280,7 @@ __device__ __forceinline__ int wave_mask(float4 q, int bx, int by) {
 // primitive's rotated content box (its template's texels above the mask
 // threshold, one texel wider: outside it no sample is pushed), of that box
 // against the wave's pixel rectangle, with one pixel of slack.  The push box
-// (wave_mask) covers the canvas axes.  An entry it rejects has no pushed
+// (box_hits) covers the canvas axes.  An entry it rejects has no pushed
 // sample in the wave, so skipping it leaves every result unchanged.
 __device__ __forceinline__ bool content_hits(const RasterArgs& A, const PrimRec& p, float cx,
                                              float cy) {
@@ -294,12 +296,14 @@ __device__ __forceinline__ bool content_hits(const RasterArgs& A, const PrimRec&
   return ((unsigned)t < (unsigned)A.ntmpl) & (fabsf(pu) <= eu) & (fabsf(pv) <= ev);
 }
 
-// Entries of staged batch `buf` that can touch this wave's pixels.
+// Entries of staged batch `buf` that can touch this wave's pixels: lane i
+// tests entry i (in the list tile's range, push box, content box).
 __device__ __forceinline__ bool entry_hits_wave(const RasterArgs& A, const PrimRec* srec,
-                                                const int* smask, int lane, int wbit, float cx,
-                                                float cy) {
-  bool in = (smask[lane] & wbit) != 0;
-  if (A.cull != nullptr) in = in & content_hits(A, srec[lane], cx, cy);
+                                                const int* sin, int lane, const WaveBox& wb,
+                                                float cx, float cy) {
+  const PrimRec& p = srec[lane];
+  bool in = (sin[lane] != 0) & box_hits(p, wb);
+  if (A.cull != nullptr) in = in & content_hits(A, p, cx, cy);
   return in;
 }
 
@@ -348,7 +352,7 @@ __device__ __forceinline__ void stage_store(const Stage& st, PrimRec (*srec)[kBa
     // vector_renderer.py:401-444: a texel fringe outside it is not drawn)
     const int tx = G.rtx, ty = G.rty;
     const bool in = (tx >= st.rg.x) & (tx <= st.rg.y) & (ty >= st.rg.z) & (ty <= st.rg.w);
-    smask[buf][ri] = in ? wave_mask(st.v, G.bx, G.by) : 0;
+    smask[buf][ri] = (in & (st.n >= 0)) ? 1 : 0;
   }
 }
 
@@ -421,7 +425,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
                                                  int* __restrict__ out_last) {
   __shared__ PrimRec srec[2][kBatch];
   __shared__ int sid[2][kBatch];
-  __shared__ int smask[2][kBatch];
+  __shared__ int smask[2][kBatch];   // 1: the entry is in the list tile's range
   __shared__ __align__(16) int salive[2][4];
   const BlockGeom G = block_geom(A, b);
   if (G.bx >= A.W || G.by >= A.H) {   // 16-px block of a tile past the canvas edge
@@ -451,9 +455,9 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
   const bool all = list_range(A, G.tile, start, end);
   DBMP_GLOBAL unsigned long long* const pm = (A.pmask != nullptr && !all) ? gptr(A.pmask) : nullptr;
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
-  const int wbit = 1 << G.w;
   const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
   const float ccy = uniform_f((float)(G.by + 8 * (G.w >> 1)) + 3.5f);
+  const WaveBox wb = wave_box(ccx, ccy);
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G);
   __syncthreads();
   int buf = 0;
@@ -465,7 +469,7 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
     const bool any_alive = wave_any(thr >= 0.f);
     unsigned long long wm = 0ull;
     if (any_alive)
-      wm = wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wbit, ccx, ccy));
+      wm = wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wb, ccx, ccy));
     Stage nxt;
     if (have_next) nxt = stage_load(A, all, base + kBatch + ri, end, part);
     unsigned long long hits = 0ull;   // entries of this batch that pushed a lane of the wave
@@ -502,15 +506,14 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
         C1 = fmaf(wgt, col[1], C1);
         C2 = fmaf(wgt, col[2], C2);
         // max(1 - a, 0) as one clamped subtraction (a >= 0: alpha_ub >= 0)
-        const float f = fminf(fmaxf(1.f - a, 0.f), 1.f);
+        float f = fminf(fmaxf(1.f - a, 0.f), 1.f);
         // a pixel stops once later layers can change neither its colour nor
         // its alpha (or alpha's derivative w.r.t. an opaque layer) by eps_T
         // (Tnz < thr; a live lane that is not pushed keeps Tnz >= eps_T,
         // since eps_T < 1, so the test needs no `pushed`)
         // nz counts exactly opaque layers (capped at 2), rare enough for a
-        // wave-uniform branch
+        // wave-uniform branch, which also keeps their zero factor out of Tnz
         const bool opq = f == 0.f;
-        Tnz = opq ? Tnz : Tnz * f;
         if (wave_any(opq)) {
           __asm__ volatile("");   // keeps this rare block a branch, not selects
           const bool first = opq & (nz == 0);
@@ -519,7 +522,9 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
           Cf2 = first ? C2 : Cf2;
           nz = opq ? min(nz + 1, 2) : nz;
           thr = ((thr >= 0.f) & (nz >= 2)) ? __builtin_inff() : thr;
+          f = opq ? 1.f : f;
         }
+        Tnz *= f;
         const bool dead = Tnz < thr;
         if (wave_any(dead)) {
           last = dead ? base - start + j + 1 : last;
@@ -564,9 +569,10 @@ __device__ __forceinline__ void raster_fwd_block(const RasterArgs& A, int b,
 }
 
 // Without colour blending the forward fits 64 VGPRs: 8 waves per SIMD, the
-// most a SIMD holds (the attribute keeps it there).
+// most a SIMD holds (the attribute keeps it there); with it, 80 VGPRs and 6
+// waves (its natural allocation, a few registers above, would give 5).
 template <int DT, bool BLEND>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(BLEND ? 1 : 8))) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(BLEND ? 6 : 8))) void k_raster_fwd(RasterArgs A, float* __restrict__ out_color,
                                                          float* __restrict__ out_alpha,
                                                          float* __restrict__ out_T,
                                                          int* __restrict__ out_last) {
@@ -595,7 +601,7 @@ struct LossArgs {
 struct BwdLds {
   PrimRec srec[2][kBatch];
   int sid[2][kBatch];
-  int smask[2][kBatch];
+  int smask[2][kBatch];   // 1: the entry is in the list tile's range
   float sred[4][kBatch][8];
   int skmax;
 };
@@ -665,9 +671,9 @@ __device__ __forceinline__ void raster_bwd_body(BwdLds& lds, const RasterArgs& A
   __syncthreads();
   if (G.lane == 0) atomicMax(&skmax, wkmax);
   const int ri = threadIdx.x >> 2, part = threadIdx.x & 3;
-  const int wbit = 1 << G.w;
   const float ccx = uniform_f((float)(G.bx + 8 * (G.w & 1)) + 3.5f);   // centre of the wave's pixels
   const float ccy = uniform_f((float)(G.by + 8 * (G.w >> 1)) + 3.5f);
+  const WaveBox wb = wave_box(ccx, ccy);
   stage_store(stage_load(A, all, start + ri, end, part), srec, sid, smask, 0, ri, part, G);
   // the forward's push masks (one word per batch, wave-uniform), loaded a
   // batch ahead
@@ -690,7 +696,7 @@ __device__ __forceinline__ void raster_bwd_body(BwdLds& lds, const RasterArgs& A
       wm = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(pm_cur >> 32)) << 32) |
            (unsigned)__builtin_amdgcn_readfirstlane((unsigned)pm_cur);
     else
-      wm = wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wbit, ccx, ccy));
+      wm = wave_ballot(entry_hits_wave(A, srec[buf], smask[buf], G.lane, wb, ccx, ccy));
     const int lim = wkmax - (base - start);
     wm = lim >= 64 ? wm : (lim <= 0 ? 0ull : wm & ((1ull << lim) - 1ull));
     Stage nxt;
@@ -839,7 +845,10 @@ __device__ __forceinline__ void raster_bwd_body(BwdLds& lds, const RasterArgs& A
   }
 }
 
-template <int DT, bool BLEND, bool LOSS>
+// Backward of block b.  MAY_MASK: the caller's lists may carry push masks
+// (single-canvas calls), so both bodies are compiled and A.pmask picks one;
+// the candidate and batched kernels never have them.
+template <int DT, bool BLEND, bool LOSS, bool MAY_MASK = true>
 __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
                                                  const float* __restrict__ gCin,
                                                  const float* __restrict__ gAin,
@@ -849,7 +858,7 @@ __device__ __forceinline__ void raster_bwd_block(const RasterArgs& A, int b,
                                                  const int* __restrict__ lastin,
                                                  float* __restrict__ grad8) {
   __shared__ BwdLds lds;
-  if (A.pmask != nullptr)
+  if (MAY_MASK && A.pmask != nullptr)
     raster_bwd_body<DT, BLEND, LOSS, true>(lds, A, b, gCin, gAin, L, Cfin, Tfin, lastin, grad8);
   else
     raster_bwd_body<DT, BLEND, LOSS, false>(lds, A, b, gCin, gAin, L, Cfin, Tfin, lastin, grad8);
@@ -929,7 +938,7 @@ __global__ __launch_bounds__(kThreads) void k_raster_bwd_cands(
   const int b = r / nb, blk = r - b * nb;
   const long px = (long)A.H * A.W;
   const LossArgs none{};
-  raster_bwd_block<DT, BLEND, false>(candidate_args(A, C, b), blk, gCin + 3 * px * b,
+  raster_bwd_block<DT, BLEND, false, false>(candidate_args(A, C, b), blk, gCin + 3 * px * b,
                                      gAin ? gAin + px * b : nullptr, none, Cfin + 3 * px * b,
                                      Tfin + px * b, lastin + px * b, grad8 + 8L * A.N * b);
 }
