@@ -78,6 +78,10 @@ def parse():
     ap.add_argument("--serve-no-graph", action="store_true")
     ap.add_argument("--serve-no-share", action="store_true",
                     help="every job holds the texel records of its own bitmaps")
+    ap.add_argument("--prewarm", type=float, default=0.5,
+                    help="seconds of untimed fitting of a throwaway copy of the workload before the "
+                         "timed runs (0: none, e.g. under a profiler, so that its per-kernel averages "
+                         "cover the timed steps)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "reference kernel for scaling HBM byte counters)")
@@ -725,7 +729,8 @@ def main():
     T, Ht, Wt = sc.templates.shape
     sdt = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[args.precision]
     runners = {"fused": run_fused, "autograd": run_autograd}
-    gpu_prewarm(sc, dev, sdt)
+    if args.prewarm > 0:
+        gpu_prewarm(sc, dev, sdt, args.prewarm)
     res = time_mode(args, runners[args.mode], sc, dev, sdt)
     other = "autograd" if args.mode == "fused" else "fused"
     res_other = time_mode(args, runners[other], sc, dev, sdt)
