@@ -59,6 +59,8 @@ struct dbmp_batch {
   long long* host_total_dev = nullptr;  // its device address
   long long cull_cap = 0;
   float4* cull = nullptr;               // per-template content boxes of every job
+  long long pm_cap = 0;
+  unsigned long long* pmask = nullptr;  // push masks of all jobs' lists (forward -> backward)
   int* counters = nullptr;              // 2 x kXcds work counters (forward, backward)
   int cus = 0;                          // compute units of the device
   int occ_fwd = 0, occ_bwd = 0;         // resident workgroups per CU of the raster kernels
@@ -305,7 +307,7 @@ int dbmp_batch_destroy(dbmp_batch* b) {
   if (b->cap_stream) (void)hipStreamSynchronize(b->cap_stream);
   void* ptrs[] = {b->djobs, b->dstate, b->dstarts, b->recs, b->counts, b->offs, b->ranges,
                   b->grad8, b->keysA, b->keysB, b->valsB, b->tidx, b->hist, b->totals, b->toffs,
-                  b->dtot, b->cull, b->counters};
+                  b->dtot, b->cull, b->counters, b->pmask};
   (void)hipDeviceSynchronize();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -376,8 +378,11 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
   if (Nt >= (1LL << 28) || Tt >= (1LL << 30))
     return fail(DBMP_E_ARG, "job set too large (%lld primitives, %lld tiles)", Nt, Tt);
   // every buffer the previous steps may still read stays alive until they end
+  // push masks: job i's lists are lists ts[i].. of the batch, so its words
+  // start ts[i] list rows in (pm_word counts the list index from there)
+  const long long pm_words = push_mask_words(pair_capacity, Tt, kBatchBin * kBatchBin);
   const bool grows = Nt > b->n_cap || pair_capacity > b->pair_alloc || Tt + 1 > b->tile_alloc ||
-                     J > b->job_cap || Ct > b->cull_cap;
+                     J > b->job_cap || Ct > b->cull_cap || pm_words > b->pm_cap;
   if (grows) HIP_TRY(hipStreamSynchronize(st));
   if ((rc = ensure_prims(b, (int)std::max<long long>(Nt, 1)))) return rc;
   if ((rc = ensure_pairs(b, pair_capacity))) return rc;
@@ -389,6 +394,13 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
   if (Ct > b->cull_cap) {
     if ((rc = grow(&b->cull, Ct))) return rc;
     b->cull_cap = Ct;
+  }
+  if (pm_words > b->pm_cap) {
+    if ((rc = grow(&b->pmask, pm_words))) {
+      b->pm_cap = 0;
+      return rc;
+    }
+    b->pm_cap = pm_words;
   }
 
   std::vector<BatchJob> hj(J);
@@ -431,6 +443,7 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.order = nullptr;
     A.cost = nullptr;
     A.cull = b->cull + cs[i];   // filled from the job's extents after the upload
+    A.pmask = b->pmask + (long long)ts[i] * (kBatchBin * kBatchBin) * 4;
     A.ntmpl = s.num_templates;
     A.bin_shift = kBatchBinShift;
     d.alpha_ub = s.alpha_upper_bound;

@@ -218,7 +218,7 @@ static __global__ __launch_bounds__(kThreads) void k_batch_bwd(const BatchJob* _
     const BatchJob& B = jobs[j];
     LossArgs L = B.L;
     L.loss = B.loss_hist + min(state[j].step, B.loss_cap - 1);
-    raster_bwd_block<DT, BLEND, true, false>(B.A, b - block_start[j], nullptr, nullptr, L, B.color, B.Tfin,
+    raster_bwd_block<DT, BLEND, true>(B.A, b - block_start[j], nullptr, nullptr, L, B.color, B.Tfin,
                                       B.last, grad8 + 8L * prim_start[j]);
   }
 }

@@ -309,13 +309,6 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   return 0;
 }
 
-// Push-mask words for lists of at most `capacity` entries in all, `lists`
-// lists and `subs` 16x16 blocks per list tile: pm_word stays below
-// floor(capacity / 64) + lists per (block of a list tile, wave).
-long long push_mask_words(long long capacity, int lists, int subs) {
-  return ((capacity >> 6) + lists + 1) * (long long)subs * 4;
-}
-
 RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps* b,
                      const dbmp_tile_lists* L, float eps) {
   RasterArgs A;

@@ -238,6 +238,13 @@ __device__ __forceinline__ long pm_word(const RasterArgs& A, const BlockGeom& G,
   return ((long)((start >> 6) + G.tile + i) * A.subs + G.sub) * 4 + G.w;
 }
 
+// Push-mask words for lists of at most `capacity` entries in all, `lists`
+// lists and `subs` 16x16 blocks per list tile (pm_word stays below
+// floor(capacity / 64) + lists per block of a list tile and wave).
+inline long long push_mask_words(long long capacity, long long lists, int subs) {
+  return ((capacity >> 6) + lists + 1) * (long long)subs * 4;
+}
+
 // Load one 16-byte quarter of the record of list entry k (thread t handles
 // quarter t&3 of record t>>2); entries past the end get an empty push box.
 __device__ __forceinline__ float4 fetch_rec_part(const RasterArgs& A, bool all, int k, int end,
