@@ -180,6 +180,8 @@ EXPORTS = {
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p]),
     "dbmp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Adam), ctypes.c_void_p]),
+    "dbmp_rescale_grads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dbmp_lane_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32]),
     "dbmp_sincos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                    ctypes.c_void_p]),
@@ -558,6 +560,15 @@ class Handle:
         check(self._L.dbmp_loss_mse(self._h, _ptr(color), _ptr(alpha), _ptr(bg), float(bg_value),
                                     _ptr(target), _ptr(grad_color), _ptr(grad_alpha), _ptr(loss_accum),
                                     _stream(color.device)))
+
+    def rescale_grads(self, grad_color, grad_alpha, scale, applied) -> None:
+        """Scale the two gradient images in place from the factor in
+        ``applied[0]`` to the one in ``scale`` (device fp32 scalars), then store
+        it in ``applied[0]``; ``applied[1]`` is the kernel's counter (zero); a
+        ratio of 1 touches nothing."""
+        check(self._L.dbmp_rescale_grads(_ptr(grad_color), grad_color.numel(), _ptr(grad_alpha),
+                                         grad_alpha.numel(), _ptr(scale), _ptr(applied),
+                                         _stream(grad_color.device)))
 
     def adam_step(self, cfg: "Adam", device) -> None:
         check(self._L.dbmp_adam_step(self._h, ctypes.byref(cfg), _stream(device)))

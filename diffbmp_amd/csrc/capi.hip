@@ -27,13 +27,17 @@ using namespace dbmp_host;
 
 namespace {
 
-__global__ void k_unpack_grads(const float* __restrict__ g8, int N, float* __restrict__ gm,
+// Unpack the (N,8) gradient rows into the five reference arrays and clear the
+// rows it read, so that the handle's rows hold zeros between calls.
+__global__ void k_unpack_grads(float* __restrict__ g8, int N, float* __restrict__ gm,
                                float* __restrict__ gr, float* __restrict__ gt,
                                float* __restrict__ gv, float* __restrict__ gc) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   const float4 a = ((const float4*)g8)[2 * n];
   const float4 b = ((const float4*)g8)[2 * n + 1];
+  ((float4*)g8)[2 * n] = make_float4(0.f, 0.f, 0.f, 0.f);
+  ((float4*)g8)[2 * n + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
   gm[2 * n] = a.x;
   gm[2 * n + 1] = a.y;
   gr[n] = a.z;
@@ -95,7 +99,12 @@ struct dbmp_rasterizer {
   int* counts = nullptr;
   long long* offs = nullptr;
   int4* ranges = nullptr;
+  // (N,8) gradient rows of the calls that unpack into the reference arrays:
+  // zeros between calls (k_unpack_grads clears what it reads); false after an
+  // allocation, or a call that failed between its raster launch and its
+  // unpack, and the next call zero-fills first
   float* grad8 = nullptr;
+  bool grad8_clean = false;
   // pair-sized scratch
   long long pair_cap = 0;
   int* keysA = nullptr;
@@ -147,6 +156,7 @@ int ensure_n(dbmp_rasterizer* h, int N) {
   if ((rc = grow(&h->offs, (long long)cap + 1 + scan_partials_len(cap)))) return rc;
   if ((rc = grow(&h->ranges, cap))) return rc;
   if ((rc = grow(&h->grad8, 8LL * cap))) return rc;
+  h->grad8_clean = false;
   h->n_cap = cap;
   return 0;
 }
@@ -185,6 +195,17 @@ constexpr int kOrderEvery = 4;
 bool uses_block_order(const dbmp_rasterizer* h) { return h->quads_used <= kOrderMaxQuadBytes; }
 const int* block_order(const dbmp_rasterizer* h, int nb) {
   return (h->order_n == nb && uses_block_order(h)) ? h->blk_order : nullptr;
+}
+
+// Before a call accumulates n values into h->grad8: zero them unless the rows
+// are known to be zero; the call's k_unpack_grads leaves them zero again.
+int grad8_begin(dbmp_rasterizer* h, long long n, hipStream_t st) {
+  if (n > 0 && !h->grad8_clean) {
+    hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(n, 256)), dim3(256), 0, st, h->grad8, n, 0.f);
+    LAUNCH_CHECK("k_fill_f32");
+  }
+  h->grad8_clean = false;
+  return 0;
 }
 
 int check_prims(const dbmp_prims* p) {
@@ -838,10 +859,7 @@ int dbmp_backward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
     if ((rc = launch_prep(h, &prims[b], nullptr, bm->num_templates, bm->height, bm->width, st,
                           nullptr, 0, h->recs + (long long)b * N)))
       return rc;
-  if (BN > 0) {
-    hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8 * BN, 256)), dim3(256), 0, st, h->grad8, 8 * BN, 0.f);
-    LAUNCH_CHECK("k_fill_f32");
-  }
+  if ((rc = grad8_begin(h, 8 * BN, st))) return rc;
   RasterArgs A = make_args(h, &prims[0], bm, &lists[0], 0.f);
   A.order = nullptr;
   A.cost = nullptr;
@@ -866,6 +884,7 @@ int dbmp_backward_batch(dbmp_rasterizer* h, int32_t B, const dbmp_prims* prims,
                        grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);
     LAUNCH_CHECK("k_unpack_grads");
   }
+  h->grad8_clean = true;
   timing_end(h, h->ev_bwd, st);
   return 0;
 }
@@ -892,10 +911,7 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
   timing_begin(h, h->ev_bwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
-  if (N > 0) {
-    hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
-    LAUNCH_CHECK("k_fill_f32");
-  }
+  if ((rc = grad8_begin(h, 8LL * N, st))) return rc;
   RasterArgs A = make_args(h, prims, bm, lists, 0.f);
   A.order = block_order(h, A.tiles * A.subs);
   const bool blend = A.cmap != nullptr;
@@ -907,6 +923,7 @@ int dbmp_backward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmap
                        grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);
     LAUNCH_CHECK("k_unpack_grads");
   }
+  h->grad8_clean = true;
   timing_end(h, h->ev_bwd, st);
   return 0;
 }
@@ -933,10 +950,7 @@ static int backward_mse_impl(dbmp_rasterizer* h, const dbmp_prims* prims, const 
   timing_begin(h, h->ev_bwd, st);
   if ((rc = ensure_quads(h, bm, st))) return rc;
   if ((rc = launch_prep(h, prims, nullptr, bm->num_templates, bm->height, bm->width, st))) return rc;
-  if (N > 0 && !rows) {
-    hipLaunchKernelGGL(k_fill_f32, dim3(ceil_div(8LL * N, 256)), dim3(256), 0, st, h->grad8, 8LL * N, 0.f);
-    LAUNCH_CHECK("k_fill_f32");
-  }
+  if (!rows && (rc = grad8_begin(h, 8LL * N, st))) return rc;
   RasterArgs A = make_args(h, prims, bm, lists, 0.f);
   A.order = block_order(h, A.tiles * A.subs);
   LossArgs L;
@@ -954,6 +968,7 @@ static int backward_mse_impl(dbmp_rasterizer* h, const dbmp_prims* prims, const 
                        grad_means2D, grad_radii, grad_rotations, grad_opacities, grad_colors);
     LAUNCH_CHECK("k_unpack_grads");
   }
+  if (!rows) h->grad8_clean = true;
   timing_end(h, h->ev_bwd, st);
   return 0;
 }
@@ -1004,6 +1019,21 @@ int dbmp_loss_mse(dbmp_rasterizer* h, const float* color, const float* alpha, co
     hipLaunchKernelGGL(k_loss_mse<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream, color, alpha,
                        bg, bg_value, target, npix, inv, grad_color, grad_alpha, loss_accum);
   LAUNCH_CHECK("k_loss_mse");
+  return 0;
+}
+
+int dbmp_rescale_grads(float* grad_color, int64_t n_color, float* grad_alpha, int64_t n_alpha,
+                       const float* scale, float* applied, void* stream) {
+  if (!scale || !applied) return fail(DBMP_E_ARG, "NULL scale");
+  if (n_color < 0 || n_alpha < 0 || (n_color > 0 && !grad_color) || (n_alpha > 0 && !grad_alpha))
+    return fail(DBMP_E_ARG, "bad gradient image");
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = std::max(n_color, n_alpha);
+  // a small grid: with a ratio of 1 (the common case) the launch is all it costs
+  const int grid = (int)std::max<long long>(1, std::min<long long>(ceil_div(n, 256), 1024));
+  hipLaunchKernelGGL(k_rescale2, dim3(grid), dim3(256), 0, st, grad_color, (long long)n_color,
+                     grad_alpha, (long long)n_alpha, scale, applied);
+  LAUNCH_CHECK("k_rescale2");
   return 0;
 }
 

@@ -162,4 +162,33 @@ static __global__ __launch_bounds__(256) void k_adam(AdamArgs A) {
   }
 }
 
+// Rescale two gradient images in place from the factor they carry to *scale:
+// x <- x * (*scale / applied[0]), applied[0] = 0 standing for 1.  Every thread
+// reads both scalars; a ratio of 1 (the usual incoming gradient of a loss)
+// leaves the images untouched and the launch is all it costs.  Otherwise the
+// last workgroup to finish (a counter in applied[1], zero between calls)
+// records the new factor: every workgroup has read applied[0] by then.
+static __global__ __launch_bounds__(256) void k_rescale2(float* __restrict__ a, long long na,
+                                                         float* __restrict__ b, long long nb,
+                                                         const float* __restrict__ scale,
+                                                         float* __restrict__ applied) {
+  const float ap = applied[0];
+  const float sc = *scale;
+  const float f = sc / (ap == 0.f ? 1.f : ap);
+  if (f == 1.f) return;   // the same on every thread
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nthr = (long long)gridDim.x * blockDim.x;
+  for (long long i = tid; i < na; i += nthr) a[i] *= f;
+  for (long long i = tid; i < nb; i += nthr) b[i] *= f;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned* cnt = reinterpret_cast<unsigned*>(applied + 1);
+    if (atomicAdd(cnt, 1u) == gridDim.x - 1) {
+      applied[0] = sc;
+      *cnt = 0u;
+    }
+  }
+}
+
 }  // namespace dbmp

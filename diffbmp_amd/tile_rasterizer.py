@@ -341,8 +341,10 @@ class CompositeMSELoss(Function):
     ``F.mse_loss`` the reference's renderer builds around the rasterizer
     (``simple_tile_renderer.py:703-707``, ``loss_functions.py:20-40``) and their
     autograd backward.  The gradient with respect to color and alpha is formed in
-    the forward pass and scaled by the incoming gradient in the backward;
-    ``target`` and ``bg`` are constants."""
+    the forward pass and scaled in place to the incoming gradient in the backward
+    (``dbmp_rescale_grads``: nothing to do for the usual incoming 1; the factor
+    already applied is tracked, so a second backward over a retained graph is
+    scaled correctly); ``target`` and ``bg`` are constants."""
 
     @staticmethod
     def forward(ctx, color, alpha, target, bg, handle, bg_value):
@@ -359,15 +361,21 @@ class CompositeMSELoss(Function):
         t = target.to(f32).contiguous()
         b = None if bg is None else bg.to(f32).contiguous()
         gC, gA = torch.empty_like(c), torch.empty_like(a)
-        loss = torch.zeros(1, dtype=f32, device=c.device)
+        # [0]: the loss; [1]: the factor the gradient images carry (0 stands for
+        # 1); [2]: the rescale's workgroup counter
+        sc = torch.zeros(3, dtype=f32, device=c.device)
+        loss = sc[0:1]
         handle.loss_mse(c, a, t, gC, gA, loss, bg=b, bg_value=bg_value)
         ctx.save_for_backward(gC, gA)
+        ctx.handle = handle
+        ctx.applied = sc[1:3]
         return loss.view(())
 
     @staticmethod
     def backward(ctx, grad_loss):
         gC, gA = ctx.saved_tensors
-        gC, gA = torch._foreach_mul((gC, gA), grad_loss)   # one multi-tensor kernel
+        g = grad_loss.to(torch.float32).reshape(1).contiguous()
+        ctx.handle.rescale_grads(gC, gA, g, ctx.applied)
         return gC, gA, None, None, None, None
 
 

@@ -261,6 +261,20 @@ int dbmp_loss_mse(dbmp_rasterizer* h, const float* color, const float* alpha, co
                   float bg_value, const float* target, float* grad_color, float* grad_alpha,
                   float* loss_accum, void* stream);
 
+/* Gradient images from dbmp_loss_mse scaled in place to an incoming gradient:
+ * x <- x * (*scale / applied[0]), then applied[0] = *scale.  scale: a device
+ * fp32 scalar; applied: two device words, the factor the images carry (fp32,
+ * start at 1 or at 0, which stands for 1) and a counter (start at 0; every
+ * call leaves it 0).  One kernel launch; a ratio of 1 leaves the images as
+ * they are, so the usual incoming gradient of a loss (1) costs no image
+ * traffic, and a second backward over the same saved images rescales them
+ * correctly.  No
+ * reference counterpart: it is the autograd backward of the fused loss
+ * (the reference multiplies by torch's incoming gradient,
+ * loss_functions.py:20-40 through torch autograd). */
+int dbmp_rescale_grads(float* grad_color, int64_t n_color, float* grad_alpha, int64_t n_alpha,
+                       const float* scale, float* applied, void* stream);
+
 /* One parameter group of dbmp_adam_step (fp32 master weights). */
 typedef struct {
   float* param;

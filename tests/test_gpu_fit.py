@@ -297,3 +297,29 @@ def test_composite_mse_loss_matches_torch(dev, with_bg):
         composite_mse_loss(h, c2, a2, target[:-1])
     with pytest.raises(ValueError):
         composite_mse_loss(h, c2, a2, target.clone().requires_grad_())
+
+
+def test_composite_mse_loss_repeated_backward(dev):
+    """The loss op's backward scales its saved gradient images in place from the
+    factor they carry to the incoming gradient: an incoming 1 leaves them as
+    they are, and backwards through a retained graph with different incoming
+    gradients (1, then 2.5, then 1 again) accumulate what torch's composite +
+    F.mse_loss accumulates."""
+    from diffbmp_amd.tile_rasterizer import composite_mse_loss
+    g = torch.Generator(dev).manual_seed(12)
+    H, W = 32, 48
+    h = Handle(H, W, 16)
+    target = torch.rand((H, W, 3), device=dev, generator=g)
+    c0 = torch.rand((H, W, 3), device=dev, generator=g)
+    a0 = torch.rand((H, W), device=dev, generator=g)
+    c1, a1 = c0.clone().requires_grad_(), a0.clone().requires_grad_()
+    c2, a2 = c0.clone().requires_grad_(), a0.clone().requires_grad_()
+    ref = F.mse_loss(c1 + (1 - a1).unsqueeze(-1), target)
+    got = composite_mse_loss(h, c2, a2, target)
+    for k, s in enumerate((1.0, 2.5, 1.0)):
+        keep = k < 2
+        (s * ref).backward(retain_graph=keep)
+        (s * got).backward(retain_graph=keep) if s != 1.0 else got.backward(retain_graph=keep)
+        torch.cuda.synchronize()
+        for x, y in ((c2, c1), (a2, a1)):
+            assert torch.allclose(x.grad, y.grad, rtol=1e-5, atol=1e-9), (k, s)
