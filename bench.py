@@ -245,10 +245,13 @@ def run_fused(args, sc, dev, sdt):
     return job, job.step, job.handle, fwd_bwd, reset
 
 
-def run_autograd(args, sc, dev, sdt, fused_loss=False):
+def run_autograd(args, sc, dev, sdt, fused_loss=False, calling_thread=False):
     """The reference-compatible TileRasterizer behind torch autograd, with the
     background composite + MSE in torch as the reference's renderer writes it,
-    or (`fused_loss`) through the one-kernel composite_mse_loss op."""
+    or (`fused_loss`) through the one-kernel composite_mse_loss op.
+    `calling_thread`: the backward runs on the thread that calls it
+    (torch.autograd.set_multithreading_enabled(False)) instead of being handed
+    to torch's per-device autograd thread."""
     import torch
     import torch.nn.functional as F
     from diffbmp_amd.tile_rasterizer import TileRasterizer
@@ -272,7 +275,11 @@ def run_autograd(args, sc, dev, sdt, fused_loss=False):
             loss = rast.composite_mse_loss(color, alpha, target)
         else:
             loss = F.mse_loss(color.float() + (1.0 - alpha.float()).unsqueeze(-1) * bg, target)
-        loss.backward()
+        if calling_thread:
+            with torch.autograd.set_multithreading_enabled(False):
+                loss.backward()
+        else:
+            loss.backward()
 
     def step():
         fwd_bwd()
@@ -735,6 +742,8 @@ def main():
     other = "autograd" if args.mode == "fused" else "fused"
     res_other = time_mode(args, runners[other], sc, dev, sdt)
     res_fl = time_mode(args, lambda a, s, d, t: run_autograd(a, s, d, t, fused_loss=True), sc, dev, sdt)
+    res_ct = time_mode(args, lambda a, s, d, t: run_autograd(a, s, d, t, fused_loss=True,
+                                                             calling_thread=True), sc, dev, sdt)
     eps_T = args.eps_t if args.eps_t is not None else 1e-6
     P, list_mean, walked = list_stats(res["handle"], sc, dev, sdt, eps_T)
 
@@ -783,6 +792,11 @@ def main():
         # (composite_mse_loss) instead of torch's elementwise and reduction ops
         "autograd_fused_loss_iters_per_s": round(args.steps / res_fl["total"], 3),
         "autograd_fused_loss_fwd_bwd_iters_per_s": round(1.0 / res_fl["fwd_bwd_s"], 3),
+        # the same with the backward on the calling thread: the default hands
+        # each backward to torch's autograd device thread, whose hand-off costs
+        # host time that this launch-heavy pass cannot hide
+        "autograd_fused_loss_calling_thread_iters_per_s": round(args.steps / res_ct["total"], 3),
+        "autograd_fused_loss_calling_thread_fwd_bwd_iters_per_s": round(1.0 / res_ct["fwd_bwd_s"], 3),
         "samples_per_s_fwd": float(f"{walked * H * W / res['fwd_s']:.4g}"),
         "pairs": P,
         "list_len_mean": round(list_mean, 2),

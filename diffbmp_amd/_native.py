@@ -103,20 +103,26 @@ class Lists:
     over these lists records which entries reached each wave's pixels and the
     backward of that forward walks only those."""
 
-    __slots__ = ("offsets", "indices", "count", "capacity", "factor", "ranges", "masks")
+    __slots__ = ("offsets", "indices", "count", "capacity", "factor", "ranges", "masks", "_c")
 
     def __init__(self, offsets, indices, count=None, capacity=0, factor=1, ranges=None, masks=None):
         self.offsets, self.indices, self.count, self.capacity = offsets, indices, count, int(capacity)
         self.factor, self.ranges, self.masks = int(factor), ranges, masks
+        self._c = None
 
     def c(self) -> TileLists:
-        idx = self.indices if self.indices.numel() else self.offsets
-        m = self.masks
-        return TileLists(self.offsets.data_ptr(), idx.data_ptr(),
-                         self.count.data_ptr() if self.count is not None else None,
-                         self.capacity, self.factor,
-                         self.ranges.data_ptr() if self.ranges is not None else None,
-                         m.data_ptr() if m is not None else None, m.numel() if m is not None else 0)
+        """The ``dbmp_tile_lists`` view of these buffers, built on first use
+        (the fields are fixed once the lists exist)."""
+        if self._c is None:
+            idx = self.indices if self.indices.numel() else self.offsets
+            m = self.masks
+            self._c = TileLists(self.offsets.data_ptr(), idx.data_ptr(),
+                                self.count.data_ptr() if self.count is not None else None,
+                                self.capacity, self.factor,
+                                self.ranges.data_ptr() if self.ranges is not None else None,
+                                m.data_ptr() if m is not None else None,
+                                m.numel() if m is not None else 0)
+        return self._c
 
 
 EXPORTS = {
@@ -246,18 +252,22 @@ def check(rc: int) -> None:
 
 
 def _ptr(t: torch.Tensor | None):
-    return None if t is None else ctypes.c_void_p(t.data_ptr())
+    """Device address of ``t`` (None for None) for a ``void*`` argument: every
+    pointer argument in ``EXPORTS`` is declared ``c_void_p``, which takes the
+    integer as is."""
+    return None if t is None else t.data_ptr()
 
 
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
 def _stream(dev: torch.device):
-    """The current torch stream of ``dev`` as a hipStream_t."""
+    """The current torch stream of ``dev`` as a hipStream_t (an integer for a
+    ``c_void_p`` argument)."""
     if _raw_stream is not None:
         idx = dev.index if dev.index is not None else torch.cuda.current_device()
-        return ctypes.c_void_p(_raw_stream(idx))
-    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        return _raw_stream(idx)
+    return torch.cuda.current_stream(dev).cuda_stream
 
 
 def require_gpu(t: torch.Tensor, name: str) -> None:
@@ -303,6 +313,7 @@ class Handle:
         # preparing its parameters may pass reuse_records while it is unchanged
         self.records_gen = 0
         self._tmpl_cache = None       # (key, converted templates, extents, version)
+        self._pm_words = {}           # (capacity, bin factor) -> push-mask words
         self._tmpl_gen = 0
 
     def __del__(self):
@@ -464,7 +475,10 @@ class Handle:
         # push masks (int64 words) | indices
         L = self.list_count(factor) + 1
         nr = 4 * max(N, 1)
-        nw = int(self._L.dbmp_push_mask_words(self._h, cap, int(factor)))
+        nw = self._pm_words.get((cap, factor))
+        if nw is None:
+            nw = self._pm_words[(cap, factor)] = int(self._L.dbmp_push_mask_words(self._h, cap,
+                                                                                 int(factor)))
         o_offs = 4 + nr
         o_pm = o_offs + ((L + 3) & ~3)
         o_idx = o_pm + 2 * nw

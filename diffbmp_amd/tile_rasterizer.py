@@ -109,6 +109,14 @@ def bin_primitives(handle: Handle, means2D, radii, rotations, bmp_sel, templates
     return handle.bin(prims, extents, tuple(templates.shape))
 
 
+def _as(t: torch.Tensor, dtype) -> torch.Tensor:
+    """``t`` in ``dtype``, contiguous; ``t`` itself when it already is (the
+    common case, without a trip through torch's dispatcher)."""
+    if t.dtype == dtype and t.is_contiguous():
+        return t
+    return t.to(dtype=dtype).contiguous()
+
+
 class _Run:
     """Everything one forward needs, normalised to contiguous device tensors."""
 
@@ -117,15 +125,13 @@ class _Run:
                  colors_orig_per_template=False):
         for t, nm in ((means2D, "means2D"), (templates, "primitive_templates")):
             _native.require_gpu(t, nm)
-        cv = lambda t: t.to(dtype=store_dtype).contiguous()
         # mixed storage: fp32 positions, radii and rotations, half-precision
         # opacities, colours and bitmaps
-        cg = (lambda t: t.to(dtype=torch.float32).contiguous()) \
-            if getattr(handle, "fp32_geometry", False) else cv
-        self.means2D, self.radii, self.rotations = cg(means2D), cg(radii), cg(rotations)
-        self.opacities, self.colors = cv(opacities), cv(colors)
+        gdt = torch.float32 if getattr(handle, "fp32_geometry", False) else store_dtype
+        self.means2D, self.radii, self.rotations = _as(means2D, gdt), _as(radii, gdt), _as(rotations, gdt)
+        self.opacities, self.colors = _as(opacities, store_dtype), _as(colors, store_dtype)
         self.templates, self.ext, self.tmpl_version = handle.prepared_templates(templates, store_dtype)
-        self.sel = bmp_sel.to(torch.int32).contiguous()
+        self.sel = _as(bmp_sel, torch.int32)
         if isinstance(c_blend, torch.Tensor):
             c_blend = float(c_blend.item())
         self.c_blend = float(c_blend)
@@ -136,7 +142,7 @@ class _Run:
             if colors_orig.dim() != 4 or colors_orig.shape[0] < need or \
                     tuple(colors_orig.shape[1:]) != (*self.templates.shape[1:], 3):
                 raise ValueError(f"colors_orig must be ({need},Ht,Wt,3), got {tuple(colors_orig.shape)}")
-            self.colors_orig = cv(colors_orig)
+            self.colors_orig = _as(colors_orig, store_dtype)
         self.eps_T = float(eps_T)
         self.blend_grad_exact = bool(getattr(handle, "blend_grad_exact", False))
         self.handle = handle
@@ -170,9 +176,9 @@ class _Run:
         out_color = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
         out_alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
         # per-pixel forward state for the backward, one allocation: T_final | last
-        state = torch.empty(2 * H * W, dtype=torch.float32, device=dev)
-        out_T = state[:H * W].view(H, W)
-        out_last = state[H * W:].view(torch.int32).view(H, W)
+        state = torch.empty((2, H, W), dtype=torch.float32, device=dev)
+        out_T = state[0]
+        out_last = state[1].view(torch.int32)
         self.handle.forward(self.prims(), self.bitmaps(), self.lists, self.eps_T,
                             out_color, out_alpha, out_T, out_last)
         self.gen = self.handle.records_gen
@@ -181,8 +187,8 @@ class _Run:
     def backward(self, grad_color, grad_alpha, out_color, out_T, out_last):
         dev, N = self.means2D.device, self.radii.shape[0]
         f32 = torch.float32
-        gC = grad_color.to(f32).contiguous()
-        gA = None if grad_alpha is None else grad_alpha.to(f32).contiguous()
+        gC = _as(grad_color, f32)
+        gA = None if grad_alpha is None else _as(grad_alpha, f32)
         g_means = torch.empty((N, 2), dtype=f32, device=dev)
         g_r = torch.empty(N, dtype=f32, device=dev)
         g_th = torch.empty(N, dtype=f32, device=dev)
@@ -357,9 +363,8 @@ class CompositeMSELoss(Function):
             if t is not None and t.requires_grad:
                 raise ValueError(f"composite_mse_loss: {nm} is not differentiable here")
         f32 = torch.float32
-        c, a = color.to(f32).contiguous(), alpha.to(f32).contiguous()
-        t = target.to(f32).contiguous()
-        b = None if bg is None else bg.to(f32).contiguous()
+        c, a, t = _as(color, f32), _as(alpha, f32), _as(target, f32)
+        b = None if bg is None else _as(bg, f32)
         gC, gA = torch.empty_like(c), torch.empty_like(a)
         # [0]: the loss; [1]: the factor the gradient images carry (0 stands for
         # 1); [2]: the rescale's workgroup counter
@@ -374,8 +379,8 @@ class CompositeMSELoss(Function):
     @staticmethod
     def backward(ctx, grad_loss):
         gC, gA = ctx.saved_tensors
-        g = grad_loss.to(torch.float32).reshape(1).contiguous()
-        ctx.handle.rescale_grads(gC, gA, g, ctx.applied)
+        # the incoming gradient of the 0-d loss: one fp32 value on the device
+        ctx.handle.rescale_grads(gC, gA, _as(grad_loss, torch.float32), ctx.applied)
         return gC, gA, None, None, None, None
 
 
