@@ -125,6 +125,42 @@ class Lists:
         return self._c
 
 
+class _PackedLists(Lists):
+    """Lists that ``Handle.bin_lazy`` lays out in one int32 buffer: pair count
+    (int64) | pad | ranges (N,4) | offsets | pad | push masks (int64 words) |
+    indices.  The C struct is built from the buffer's address; the tensor
+    views of the fields are made only when one is asked for."""
+
+    __slots__ = ("_buf", "_lay", "_views")
+
+    def __init__(self, buf, capacity, factor, lay):
+        # lay: (offsets start, offsets length, masks start, indices start) in int32 words
+        self._buf, self._lay, self._views = buf, lay, None
+        self.capacity, self.factor, self._c = int(capacity), int(factor), None
+
+    def _view(self):
+        if self._views is None:
+            b = self._buf
+            o_offs, n_offs, o_pm, o_idx = self._lay
+            self._views = (b[o_offs:o_offs + n_offs], b[o_idx:], b[0:2].view(torch.int64),
+                           b[4:o_offs].view(-1, 4), b[o_pm:o_idx].view(torch.int64))
+        return self._views
+
+    offsets = property(lambda self: self._view()[0])
+    indices = property(lambda self: self._view()[1])
+    count = property(lambda self: self._view()[2])
+    ranges = property(lambda self: self._view()[3])
+    masks = property(lambda self: self._view()[4])
+
+    def c(self) -> TileLists:
+        if self._c is None:
+            a = self._buf.data_ptr()
+            o_offs, _, o_pm, o_idx = self._lay
+            self._c = TileLists(a + 4 * o_offs, a + 4 * o_idx, a, self.capacity, self.factor, a + 16,
+                                a + 4 * o_pm, (o_idx - o_pm) // 2)
+        return self._c
+
+
 EXPORTS = {
     "dbmp_version": (ctypes.c_int, []),
     "dbmp_last_error": (ctypes.c_char_p, []),
@@ -392,16 +428,23 @@ class Handle:
         ``factor`` > 1, optional for per-tile lists): after a pair overflow the
         raster kernels walk every primitive and use it only inside that range."""
         T, Ht, Wt = tmpl_shape
-        self.records_gen += 1
         if factor > 1 or ranges is not None:
-            check(self._L.dbmp_bin_tiles(self._h, ctypes.byref(prims), _ptr(ext), int(T), int(Ht),
-                                         int(Wt), int(factor), _ptr(tile_offsets),
-                                         _ptr(tile_indices), int(capacity), _ptr(pair_count),
-                                         _ptr(ranges), _stream(ext.device)))
+            self._bin_tiles(prims, ext, tmpl_shape, factor, _ptr(tile_offsets), _ptr(tile_indices),
+                            capacity, _ptr(pair_count), _ptr(ranges))
             return
+        self.records_gen += 1
         check(self._L.dbmp_bin(self._h, ctypes.byref(prims), _ptr(ext), int(T), int(Ht), int(Wt),
                                _ptr(tile_offsets), _ptr(tile_indices), int(capacity),
                                _ptr(pair_count), _stream(ext.device)))
+
+    def _bin_tiles(self, prims: Prims, ext: torch.Tensor, tmpl_shape, factor: int, offsets: int,
+                   indices: int, capacity: int, pair_count: int, ranges: int) -> None:
+        """dbmp_bin_tiles on device addresses."""
+        T, Ht, Wt = tmpl_shape
+        self.records_gen += 1
+        check(self._L.dbmp_bin_tiles(self._h, ctypes.byref(prims), ext.data_ptr(), int(T), int(Ht),
+                                     int(Wt), int(factor), offsets, indices, int(capacity),
+                                     pair_count, ranges, _stream(ext.device)))
 
     @staticmethod
     def _ranges(prims: Prims, dev):
@@ -483,20 +526,16 @@ class Handle:
         o_pm = o_offs + ((L + 3) & ~3)
         o_idx = o_pm + 2 * nw
         buf = torch.empty(o_idx + cap, dtype=torch.int32, device=dev)
-        cnt = buf[0:2].view(torch.int64)
-        rg = buf[4:o_offs].view(-1, 4)
-        offs = buf[o_offs:o_offs + L]
-        pm = buf[o_pm:o_idx].view(torch.int64)
-        idx = buf[o_idx:]
-        self.bin_async(prims, ext, tmpl_shape, cap, offs, idx, cnt, factor, rg)
+        a = buf.data_ptr()
+        self._bin_tiles(prims, ext, tmpl_shape, factor, a + 4 * o_offs, a + 4 * o_idx, cap, a, a + 16)
         if self._pending is None and self._lazy_calls % self.READBACK_EVERY == 0:
             if self._readback is None:
                 self._readback = torch.empty(1, dtype=torch.int64, pin_memory=True)
                 self._readback_ev = torch.cuda.Event()
-            self._readback.copy_(cnt, non_blocking=True)
+            self._readback.copy_(buf[0:2].view(torch.int64), non_blocking=True)
             self._readback_ev.record(torch.cuda.current_stream(dev))
             self._pending = self._readback_ev
-        return Lists(offs, idx, cnt, cap, factor, rg, pm)
+        return _PackedLists(buf, cap, factor, (o_offs, L, o_pm, o_idx))
 
     def forward(self, prims: Prims, bm: Bitmaps, lists: Lists, eps_T: float, out_color, out_alpha,
                 out_T, out_last) -> None:

@@ -175,10 +175,9 @@ class _Run:
         dev, H, W = self.means2D.device, self.handle.H, self.handle.W
         out_color = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
         out_alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
-        # per-pixel forward state for the backward, one allocation: T_final | last
-        state = torch.empty((2, H, W), dtype=torch.float32, device=dev)
-        out_T = state[0]
-        out_last = state[1].view(torch.int32)
+        # per-pixel forward state for the backward
+        out_T = torch.empty((H, W), dtype=torch.float32, device=dev)
+        out_last = torch.empty((H, W), dtype=torch.int32, device=dev)
         self.handle.forward(self.prims(), self.bitmaps(), self.lists, self.eps_T,
                             out_color, out_alpha, out_T, out_last)
         self.gen = self.handle.records_gen
