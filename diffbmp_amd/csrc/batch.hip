@@ -428,6 +428,7 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     A.Ht = s.tmpl_height;
     A.Wt = s.tmpl_width;
     A.HWm1f = (float)(s.tmpl_height * s.tmpl_width - 1);
+    A.Wtf = (float)s.tmpl_width;
     A.hsx = 0.5f * (float)(s.tmpl_width - 1);
     A.hsy = 0.5f * (float)(s.tmpl_height - 1);
     A.H = s.height;

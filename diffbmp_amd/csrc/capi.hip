@@ -356,6 +356,7 @@ RasterArgs make_args(dbmp_rasterizer* h, const dbmp_prims* p, const dbmp_bitmaps
   A.Ht = b->height;
   A.Wt = b->width;
   A.HWm1f = (float)(b->height * b->width - 1);
+  A.Wtf = (float)b->width;
   A.hsx = 0.5f * (float)(b->width - 1);
   A.hsy = 0.5f * (float)(b->height - 1);
   A.H = h->H;

@@ -54,6 +54,7 @@ struct RasterArgs {
                          // 0: the reference's (no (1 - c_blend), no colour-map term)
   int N, Ht, Wt;
   float HWm1f;           // Ht*Wt - 1: last texel of one template, as a float
+  float Wtf;             // Wt as a float (a scalar operand of the texel-index fma)
   float hsx, hsy;        // (Wt-1)/2, (Ht-1)/2
   int H, W, ts, tilesX, tiles, subs;   // ts: list tile; subs = (ts/16)^2 blocks per tile
   float eps;
@@ -400,7 +401,7 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   // that is not `ok` reads some valid quad and its sample is discarded
   // texel index within the template in float (exact below 2^24), clamped so
   // that NaN or out-of-range coordinates still address a texel of the template
-  const float fl = __builtin_amdgcn_fmed3f(fmaf(fy, (float)A.Wt, fx), 0.f, A.HWm1f);
+  const float fl = __builtin_amdgcn_fmed3f(fmaf(fy, A.Wtf, fx), 0.f, A.HWm1f);
   // with colour maps (small, minified primitives are typical there) a lane
   // that cannot push reads the template's first texel, so that idle lanes do
   // not spread cache lines over the bitmap
