@@ -15,7 +15,10 @@ jobs start from the original parameters.
 
 ``roofline``: the dominant kernel (backward raster) -- its compulsory HBM
 bytes per launch divided by its mean duration, measured live with hipEvents
-on the launch stream, against 8 TB/s.  ``cpu_baseline``: the reference
+on the launch stream, against 8 TB/s.  ``issue_roofline`` and
+``lds_roofline``: the same kernel's wave64 VALU instructions and LDS bytes per
+launch (from the committed PMC passes under ``profiles/``) over the same live
+duration, against 614 G instructions/s and 78.6 TB/s.  ``cpu_baseline``: the reference
 algorithm in fp32 on the host cores (OpenMP build of the oracle): full
 optimisation steps of the same workload (median of 3), and the plain
 workload's 100-step fit (``plain_fit``, also run on the GPU).
@@ -39,6 +42,8 @@ HBM_PEAK = 8.0e12
 # CU per cycle (4 SIMDs of 16 lanes, 4 cycles per wave64 instruction, no
 # packed fp32)
 VALU_ISSUE_PEAK = 256 * 2.4e9
+# LDS bytes per second: 128 B per CU per cycle (32 banks x 4 B), 256 CUs, 2.4 GHz
+LDS_PEAK = 128 * 256 * 2.4e9
 
 
 def parse():
@@ -821,6 +826,20 @@ def main():
                                      "frac": round(ach / VALU_ISSUE_PEAK, 4), "kernel": bwd_kernel,
                                      "valu_instructions": vk["SQ_INSTS_VALU"],
                                      "source": "profiles/pmc_valu_scale_fp32.json"}
+    # LDS: bytes the backward moves through LDS per launch (committed
+    # SQ_LDS_IDX_ACTIVE pass: 128 B per conflict-free active cycle) over the
+    # live duration, against 128 B/clk per CU
+    lprof = os.path.join(HERE, "profiles", "pmc_lds_scale_fp32.json")
+    if args.precision == "fp32" and os.path.exists(lprof):
+        with open(lprof) as f:
+            lk = json.load(f)["kernels"].get(bwd_kernel)
+        if lk:
+            ach = lk["lds_bytes"] / res["bwd_s"]
+            out["lds_roofline"] = {"bound": "lds", "achieved": round(ach / 1e9, 1),
+                                   "peak": LDS_PEAK / 1e9, "unit": "GB/s",
+                                   "frac": round(ach / LDS_PEAK, 4), "kernel": bwd_kernel,
+                                   "lds_bytes": lk["lds_bytes"],
+                                   "source": "profiles/pmc_lds_scale_fp32.json"}
     if args.psnr_steps:
         out["psnr"] = psnr_compare(args, sc, dev, sdt)
     if not args.no_precision:
