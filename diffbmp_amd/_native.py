@@ -236,6 +236,7 @@ EXPORTS = {
     "dbmp_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
                                          ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     "dbmp_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "dbmp_batch_bitmap_bytes": (ctypes.c_int64, [ctypes.c_void_p, ctypes.POINTER(Bitmaps)]),
     "dbmp_batch_prepare_bitmaps": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Bitmaps),
                                                   ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_void_p]),

@@ -92,15 +92,16 @@ void drop_graph(dbmp_batch* b) {
   b->graph = nullptr;
 }
 
-int check_job(const dbmp_job& j, int i) {
+int check_job(const dbmp_job& j, int i, int dtype) {
   if (j.height <= 0 || j.width <= 0) return fail(DBMP_E_ARG, "job %d: canvas must be positive", i);
   if (j.num_primitives < 0) return fail(DBMP_E_ARG, "job %d: num_primitives < 0", i);
   if (!(j.alpha_upper_bound >= 0.f))
     return fail(DBMP_E_ARG, "job %d: alpha_upper_bound must be >= 0", i);
   if (j.num_templates <= 0 || j.tmpl_height < 2 || j.tmpl_width < 2)
     return fail(DBMP_E_ARG, "job %d: bad template shape", i);
-  if ((long long)j.num_templates * j.tmpl_height * j.tmpl_width >= (1LL << 28))
-    return fail(DBMP_E_ARG, "job %d: templates hold more than 2^28 texels", i);
+  if (texel_record_bytes(j.num_templates, j.tmpl_height, j.tmpl_width, dtype,
+                         j.quads_with_color != 0) >= (1LL << 32))
+    return fail(DBMP_E_ARG, "job %d: the texel records reach 4 GiB", i);
   if (!j.quads || !j.ext) return fail(DBMP_E_ARG, "job %d: prepared bitmaps missing", i);
   if (j.num_primitives > 0 && (!j.bmp_sel || !j.params || !j.exp_avg || !j.exp_avg_sq || !j.store))
     return fail(DBMP_E_ARG, "job %d: a parameter pointer is NULL", i);
@@ -319,6 +320,12 @@ int dbmp_batch_destroy(dbmp_batch* b) {
   return 0;
 }
 
+int64_t dbmp_batch_bitmap_bytes(const dbmp_batch* b, const dbmp_bitmaps* bm) {
+  if (!b || !bm || bm->num_templates <= 0 || bm->height < 2 || bm->width < 2) return -1;
+  return texel_record_bytes(bm->num_templates, bm->height, bm->width, b->dtype,
+                            bm->colors_orig_mode == 2 && bm->colors_orig);
+}
+
 int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext_out,
                                void* quads_out, void* stream) {
   if (!b) return fail(DBMP_E_ARG, "batch is NULL");
@@ -327,8 +334,10 @@ int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext
   if (bm->num_templates <= 0 || bm->height < 2 || bm->width < 2)
     return fail(DBMP_E_ARG, "templates must be (T>0, H>=2, W>=2)");
   if (bm->dtype != b->dtype) return fail(DBMP_E_ARG, "template dtype differs from the batch's");
-  if ((long long)bm->num_templates * bm->height * bm->width >= (1LL << 28))
-    return fail(DBMP_E_ARG, "templates hold more than 2^28 texels");
+  if (texel_record_bytes(bm->num_templates, bm->height, bm->width, bm->dtype,
+                         bm->colors_orig_mode == 2 && bm->colors_orig) >= (1LL << 32))
+    return fail(DBMP_E_ARG, "the texel records of %d templates of %dx%d reach 4 GiB",
+                bm->num_templates, bm->height, bm->width);
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(bm->num_templates), block(256);
   if (bm->dtype == DBMP_F32)
@@ -354,7 +363,7 @@ int dbmp_batch_set_jobs(dbmp_batch* b, const dbmp_job* jobs, int32_t num_jobs,
     return fail(DBMP_E_ARG, "pair_capacity must be in [0, 2^31)");
   int rc;
   for (int i = 0; i < num_jobs; ++i)
-    if ((rc = check_job(jobs[i], i))) return rc;
+    if ((rc = check_job(jobs[i], i, b->dtype))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const int J = num_jobs;
   std::vector<int> ps(J + 1), ts(J + 1), bs(J + 1);

@@ -140,7 +140,7 @@ __device__ __forceinline__ void prep_core(float x, float y, float r, float th, f
   p.dav = alpha_ub * p.sop * (1.f - p.sop);
   const bool valid = (t >= 0 && t < T);
   p.tmpl = valid ? t : -1;
-  p.tbase = valid ? t * Ht * Wt : 0;
+  p.tbase = valid ? t * (Ht + 1) * Wt : 0;   // Ht + 1 rows of Wt entries per template
   if (!valid) {   // never drawn: an empty push box rejects every pixel
     p.hw = -1.f;
     p.hh = -1.f;

@@ -226,8 +226,11 @@ int check_bitmaps(const dbmp_bitmaps* b) {
     return fail(DBMP_E_ARG, "templates must be (T>0, H>=2, W>=2), got (%d,%d,%d)",
                 b->num_templates, b->height, b->width);
   if (b->dtype < DBMP_F32 || b->dtype > DBMP_BF16) return fail(DBMP_E_ARG, "bad bitmap dtype %d", b->dtype);
-  if ((long long)b->num_templates * b->height * b->width >= (1LL << 28))
-    return fail(DBMP_E_ARG, "templates hold more than 2^28 texels");
+  // the raster kernels address the texel records with 32-bit byte offsets
+  const bool rgba = b->colors_orig_mode == 2 && b->colors_orig != nullptr && b->c_blend > 0.f;
+  if (texel_record_bytes(b->num_templates, b->height, b->width, b->dtype, rgba) >= (1LL << 32))
+    return fail(DBMP_E_ARG, "the texel records of %d templates of %dx%d reach 4 GiB",
+                b->num_templates, b->height, b->width);
   if (b->colors_orig_mode < 0 || b->colors_orig_mode > 2)
     return fail(DBMP_E_ARG, "bad colors_orig_mode %d", b->colors_orig_mode);
   if (b->colors_orig_mode != 0 && !b->colors_orig) return fail(DBMP_E_ARG, "colors_orig is NULL");
@@ -282,22 +285,21 @@ bool wants_rgba(const dbmp_bitmaps* b) {
 }
 
 int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
-  const long long tot = (long long)b->num_templates * b->height * b->width;
   const bool rgba = wants_rgba(b);
-  const size_t esz = rgba ? rgba_bytes(b->dtype) : quad_bytes(b->dtype);
+  const size_t bytes = (size_t)texel_record_bytes(b->num_templates, b->height, b->width, b->dtype, rgba);
   if (b->version >= 0 && h->q_src == b->templates && h->q_version == b->version &&
       h->q_T == b->num_templates && h->q_Ht == b->height && h->q_Wt == b->width &&
       h->q_dt == b->dtype && h->q_rgba == rgba &&
       (!rgba || (b->colors_version >= 0 && h->q_csrc == b->colors_orig &&
                  h->q_cversion == b->colors_version)))
     return 0;
-  if ((size_t)tot * esz > h->quads_bytes) {
+  if (bytes > h->quads_bytes) {
     if (h->quads) (void)hipFree(h->quads);
     h->quads = nullptr;
     h->quads_bytes = 0;
-    hipError_t e = hipMalloc(&h->quads, (size_t)tot * esz);
+    hipError_t e = hipMalloc(&h->quads, bytes);
     if (e != hipSuccess) return fail(DBMP_E_ALLOC, "hipMalloc quads: %s", hipGetErrorString(e));
-    h->quads_bytes = (size_t)tot * esz;
+    h->quads_bytes = bytes;
   }
   int rc = rgba ? launch_build_rgba(b, h->quads, st) : launch_build_quads(b, h->quads, st);
   if (rc) return rc;
@@ -326,7 +328,7 @@ int ensure_quads(dbmp_rasterizer* h, const dbmp_bitmaps* b, hipStream_t st) {
   h->q_rgba = rgba;
   h->q_csrc = b->colors_orig;
   h->q_cversion = b->colors_version;
-  h->quads_used = (size_t)tot * esz;
+  h->quads_used = bytes;
   return 0;
 }
 
@@ -690,8 +692,8 @@ int dbmp_forward_noalloc(int image_height, int image_width, int tile_size, float
     return fail(DBMP_E_ARG, "dbmp_forward_noalloc: %d templates exceed its maximum of %d",
                 bm->num_templates, kNoAllocMaxTemplates);
   const bool rgba = wants_rgba(bm);
-  const size_t qbytes = (size_t)bm->num_templates * bm->height * bm->width *
-                        (rgba ? rgba_bytes(bm->dtype) : quad_bytes(bm->dtype));
+  const size_t qbytes = (size_t)texel_record_bytes(bm->num_templates, bm->height, bm->width,
+                                                   bm->dtype, rgba);
   if (qbytes > kNoAllocQuadBytes)
     return fail(DBMP_E_ARG, "dbmp_forward_noalloc: texel records of %zu bytes exceed its maximum of %zu",
                 qbytes, kNoAllocQuadBytes);

@@ -22,7 +22,7 @@ struct alignas(16) PrimRec {
   float mx, my, hw, hh;      // centre; push-box half extents (tile_forward.cu:195-210)
   float c, s, rinv, abase;   // cos, sin (dbmp_sincosf), 1/r, alpha_ub * sigmoid(v)
   float cr, cg, cb;          // sigmoid(colour logits)
-  int tbase;                 // tmpl * Ht * Wt: first texel quad of the template
+  int tbase;                 // tmpl * (Ht + 1) * Wt: first texel entry of the template
   int tmpl;                  // template index, -1 => never pushed
   float r;                   // radius
   float sop;                 // sigmoid(opacity logit)
@@ -124,12 +124,25 @@ template <> struct QuadT<kBF16> {
   }
 };
 
-// RGBA texel records of a template set with per-template colour maps: entry
-// (t, y, x) holds the template's bilinear neighbourhood (the alpha quad of
-// QuadT, zero outside the bitmap) followed by the colour map's clamp-to-edge
-// neighbourhood: RGB of (x,y), (x1,y), (x,y1), (x1,y1), x1 = min(x+1, W-1),
-// y1 = min(y+1, H-1).  16 values, one aligned 64-byte (fp32) or 32-byte
-// (fp16 / bf16) record: a sample's alpha and colour arrive in one cache line.
+// RGBA texel records of a template set with per-template colour maps.  Every
+// template holds Ht + 1 rows of Wt entries (as the alpha quads do); `rs` is
+// the row stride Wt.
+// RGBA texel records of a template set with per-template colour maps.  Every
+// template holds Ht + 1 rows of Wt entries (as the alpha quads do); `rs` is
+// the row stride Wt.
+// fp32: entry (t, y, x) holds the template's bilinear neighbourhood (the alpha
+// quad of QuadT, zero outside the bitmap) followed by the colour map's
+// clamp-to-edge neighbourhood: RGB of (x,y), (x1,y), (x,y1), (x1,y1),
+// x1 = min(x+1, W-1), y1 = min(y+1, H-1) -- 16 values, one aligned 64-byte
+// record; the padding row is zero.
+// fp16 / bf16: entry (t, y, x) holds one row of the neighbourhood, 8 values in
+// one aligned 16-byte record: alpha of (x,y) and (x+1,y) (zero past the
+// bitmap), then RGB of (x,y) and (x1,y).  The padding row y = Ht holds alpha
+// 0 and the colours of row Ht-1 (clamp-to-edge).  A sample reads the entries
+// of rows y and y+1 (two loads, as a 32-byte record of the whole
+// neighbourhood would take): each texel is stored twice, not four times, so
+// the colour-map workloads keep half the bytes in cache.  (The same layout in
+// fp32 made the backward slower: its colour reads became four loads.)
 template <int DT> struct RecT;
 template <> struct RecT<kF32> {
   // Only the record's address travels with the sample; its colour part is
@@ -138,14 +151,15 @@ template <> struct RecT<kF32> {
   // or a per-primitive map in global memory) into one load through a
   // selected pointer, which put the record in private (scratch) memory.
   struct Col { const DBMP_GLOBAL char* rec; };
-  __device__ static __forceinline__ void get(const void* base, unsigned i, float& a, float& b,
-                                             float& c, float& d, Col& col) {
+  __device__ static __forceinline__ void get(const void* base, unsigned i, unsigned /*rs*/, float& a,
+                                             float& b, float& c, float& d, Col& col) {
     const DBMP_GLOBAL char* q = gbytes(base) + (i << 6);
     const float4 r0 = ld_f4(q);
     a = r0.x; b = r0.y; c = r0.z; d = r0.w;
     col.rec = q;
   }
-  __device__ static __forceinline__ void colours(const Col& col, float v[12]) {
+  // 12 values: RGB of (x,y), (x1,y), (x,y1), (x1,y1)
+  __device__ static __forceinline__ void colours(const Col& col, unsigned /*rs*/, float v[12]) {
     const float4 r1 = ld_f4(col.rec + 16), r2 = ld_f4(col.rec + 32), r3 = ld_f4(col.rec + 48);
     v[0] = r1.x; v[1] = r1.y; v[2] = r1.z; v[3] = r1.w;
     v[4] = r2.x; v[5] = r2.y; v[6] = r2.z; v[7] = r2.w;
@@ -157,14 +171,14 @@ struct RecHalf {
   struct Col { uint32_t v[6]; };
   __device__ static __forceinline__ float lo(uint32_t v) { return DT == kF16 ? f16_lo(v) : bf16_lo(v); }
   __device__ static __forceinline__ float hi(uint32_t v) { return DT == kF16 ? f16_hi(v) : bf16_hi(v); }
-  __device__ static __forceinline__ void get(const void* base, unsigned i, float& a, float& b,
-                                             float& c, float& d, Col& col) {
-    const DBMP_GLOBAL char* q = gbytes(base) + (i << 5);
-    const uint4 r0 = ld_u4(q), r1 = ld_u4(q + 16);
-    a = lo(r0.x); b = hi(r0.x); c = lo(r0.y); d = hi(r0.y);
-    col.v[0] = r0.z; col.v[1] = r0.w; col.v[2] = r1.x; col.v[3] = r1.y; col.v[4] = r1.z; col.v[5] = r1.w;
+  __device__ static __forceinline__ void get(const void* base, unsigned i, unsigned rs, float& a,
+                                             float& b, float& c, float& d, Col& col) {
+    const uint4 r0 = ld_u4(gbytes(base) + (i << 4)), r1 = ld_u4(gbytes(base) + ((i + rs) << 4));
+    a = lo(r0.x); b = hi(r0.x); c = lo(r1.x); d = hi(r1.x);
+    // RGB (x,y), (x1,y) from row y; (x,y1), (x1,y1) from row y + 1
+    col.v[0] = r0.y; col.v[1] = r0.z; col.v[2] = r0.w; col.v[3] = r1.y; col.v[4] = r1.z; col.v[5] = r1.w;
   }
-  __device__ static __forceinline__ void colours(const Col& col, float v[12]) {
+  __device__ static __forceinline__ void colours(const Col& col, unsigned /*rs*/, float v[12]) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = (k & 1) ? hi(col.v[k >> 1]) : lo(col.v[k >> 1]);
   }

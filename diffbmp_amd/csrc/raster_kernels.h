@@ -99,9 +99,9 @@ struct CCorners {
 };
 
 // Colour corners of a sample at texel coordinates (cx, cy): from the RGBA
-// record the sample's gather already brought in (cmap_mode 2; a pushed sample
-// has x0 = floor(tx), the record's texel), or gathered from per-primitive
-// maps (cmap_mode 1).
+// records of the sample's gather (cmap_mode 2; a pushed sample has
+// x0 = floor(tx), the records' texel), or gathered from per-primitive maps
+// (cmap_mode 1).
 template <int DT>
 __device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const PrimRec& p, int n,
                                                  float cx, float cy,
@@ -114,7 +114,7 @@ __device__ __forceinline__ CCorners cmap_corners(const RasterArgs& A, const Prim
   k.fy = cy - (float)y0;
   if (A.cmap_mode == 2) {
     float rc[12];
-    RecT<DT>::colours(raw, rc);
+    RecT<DT>::colours(raw, (unsigned)A.Wt, rc);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
       k.c00[ch] = rc[ch];
@@ -408,7 +408,7 @@ __device__ __forceinline__ Sample<DT> eval_sample(const RasterArgs& A, const Pri
   const unsigned idx = (unsigned)(p.tbase + ((!BLEND || ok) ? (int)fl : 0));
   float v00, v01, v10, v11;
   if (BLEND && A.cmap_mode == 2)
-    RecT<DT>::get(A.quads, idx, v00, v01, v10, v11, r.craw);
+    RecT<DT>::get(A.quads, idx, (unsigned)A.Wt, v00, v01, v10, v11, r.craw);
   else
     Q::get((const typename Q::type*)A.quads, idx, v00, v01, v10, v11);
   const float wx = r.tx - fx, wy = r.ty - fy;

@@ -31,7 +31,7 @@ from . import _native
 from ._native import Bitmaps, Job, check
 
 _QUAD_BYTES = {torch.float32: 16, torch.float16: 8, torch.bfloat16: 8}
-_RGBA_BYTES = {torch.float32: 64, torch.float16: 32, torch.bfloat16: 32}
+_RGBA_BYTES = {torch.float32: 64, torch.float16: 16, torch.bfloat16: 16}
 
 
 @dataclass
@@ -63,7 +63,7 @@ def job_bytes(spec: JobSpec, store_dtype=torch.float16) -> int:
     es = torch.tensor([], dtype=store_dtype).element_size()
     b = N * 8 * 4 * 3 + N * 8 * es + N * 4          # master, moments, store, sel
     rec = _RGBA_BYTES[store_dtype] if spec.c_blend > 0 else _QUAD_BYTES[store_dtype]
-    b += T * Ht * Wt * (es + 3 * es + rec) + T * 16     # bitmaps, maps, texel records, extents
+    b += T * Ht * Wt * (es + 3 * es) + T * (Ht + 1) * Wt * rec + T * 16   # bitmaps, maps, records, extents
     b += H * W * (3 + 3 + 1 + 1 + 1) * 4             # target, colour, alpha, T, last
     b += spec.steps * 4                              # loss history
     return int(b)
@@ -175,10 +175,12 @@ def prepare_bitmaps(batch: "BatchEngine", templates, colors_orig, blend: bool):
     dev, sdt = batch.device, batch.store_dtype
     T, Ht, Wt = templates.shape
     ext = torch.empty((T, 4), dtype=torch.float32, device=dev)
-    rec = _RGBA_BYTES[sdt] if blend else _QUAD_BYTES[sdt]
-    quads = torch.empty(T * Ht * Wt * rec, dtype=torch.uint8, device=dev)
     bm = Bitmaps(templates.data_ptr(), colors_orig.data_ptr() if blend else None, T, Ht, Wt,
                  _native._DT[sdt], 2 if blend else 0, 0.0, -1, 0, -1)
+    nbytes = int(batch._L.dbmp_batch_bitmap_bytes(batch._b, ctypes.byref(bm)))
+    if nbytes < 0:
+        raise ValueError(f"bad template set {(T, Ht, Wt)}")
+    quads = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
         check(batch._L.dbmp_batch_prepare_bitmaps(batch._b, ctypes.byref(bm), _native._ptr(ext),
                                                  _native._ptr(quads), _native._stream(dev)))

@@ -202,8 +202,8 @@ int dbmp_forward(dbmp_rasterizer* h, const dbmp_prims* prims, const dbmp_bitmaps
  * the reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Limits: at most 65536 primitives, 4096 templates and 16 MiB
- * of texel records (T*Ht*Wt*16 bytes for fp32 alpha bitmaps, *8 for fp16 /
- * bf16, *64 / *32 with per-template colour maps); larger calls return
+ * of texel records (T*(Ht+1)*Wt*16 bytes for fp32 alpha bitmaps, *8 for
+ * fp16 / bf16, *64 / *16 with per-template colour maps); larger calls return
  * DBMP_E_ARG.  Lists as for dbmp_forward, except that lists with a pair count
  * or bin_factor > 1 must carry their ranges.  The array keeps the last call's
  * contents and every call rewrites what it reads; calls that overlap on one
@@ -380,10 +380,12 @@ int dbmp_batch_create(dbmp_batch** out, int32_t storage_dtype, float beta1, floa
 int dbmp_batch_destroy(dbmp_batch* b);
 
 /* Template extents (T,4) and texel records of one template set, for
- * dbmp_job: T*Ht*Wt alpha quads of 16 bytes (fp32 storage) or 8 bytes, or,
- * when bm carries per-template colour maps (colors_orig_mode 2), T*Ht*Wt RGBA
- * records of 64 / 32 bytes holding each texel's alpha and colour
- * neighbourhoods (set quads_with_color in the job). */
+ * dbmp_job, into quads_out (dbmp_batch_bitmap_bytes bytes): T*(Ht+1)*Wt alpha
+ * quads of 16 bytes (fp32 storage) or 8 bytes, or, when bm carries
+ * per-template colour maps (colors_orig_mode 2), T*(Ht+1)*Wt RGBA records of
+ * 64 / 16 bytes holding the texels' alpha and colour neighbourhoods (set
+ * quads_with_color in the job).  Sets of 4 GiB or more are rejected. */
+int64_t dbmp_batch_bitmap_bytes(const dbmp_batch* b, const dbmp_bitmaps* bm);
 int dbmp_batch_prepare_bitmaps(dbmp_batch* b, const dbmp_bitmaps* bm, float* ext_out,
                                void* quads_out, void* stream);
 

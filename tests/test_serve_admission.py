@@ -79,8 +79,9 @@ def run(specs, budget, max_jobs=8):
 def test_job_bytes_counts_every_buffer():
     s = spec("a", 256, 1000, 4, 200, 0.0)
     b16 = SV.job_bytes(s, torch.float16)
-    want = 1000 * 8 * 4 * 3 + 1000 * 8 * 2 + 1000 * 4 + 4 * 64 * 64 * (2 + 6 + 32) + 4 * 16 \
-        + 256 * 256 * 9 * 4 + 200 * 4
+    # RGBA records: Ht + 1 rows of Wt 16-byte entries per template
+    want = 1000 * 8 * 4 * 3 + 1000 * 8 * 2 + 1000 * 4 + 4 * 64 * 64 * (2 + 6) + 4 * 65 * 64 * 16 \
+        + 4 * 16 + 256 * 256 * 9 * 4 + 200 * 4
     assert b16 == want
     assert SV.job_bytes(s, torch.float32) > b16
 
