@@ -114,3 +114,40 @@ def test_tile_sizes_the_kernels_cannot_take(dev, ts):
     launched."""
     with pytest.raises((ValueError, _native.NativeError)):
         TileRasterizer(64, 64, ts, 0.0, 1.0, 151, 10)
+
+
+@pytest.mark.parametrize("dtype,exact", [(torch.float16, False), (torch.float16, True),
+                                         (torch.bfloat16, False), (torch.float32, True)])
+def test_colour_map_samples_on_the_last_texel(dev, dtype, exact):
+    """Samples that land exactly on a bitmap's last column and row (u = 1 or
+    v = 1: r = 4, theta = 0, integer centres) read the right neighbour past the
+    bitmap (alpha 0) and the colour map's clamp-to-edge neighbours: the
+    padding of the RGBA records.  Images and gradients (both colour-gradient
+    modes) against the oracle on the same storage-rounded inputs."""
+    from helpers import rounded
+    rng = np.random.default_rng(21)
+    Ht = Wt = 5
+    N = 6
+    means = np.array([[10, 10], [30, 12], [19, 27], [41, 31], [10, 36], [33, 40]], np.float32)
+    T = 2
+    # hw = Wt/2 * r/(Wt/2) = r: the push box reaches exactly the u = +-1 pixels
+    templates = np.stack([np.ones((Ht, Wt)), 0.25 + 0.75 * rng.random((Ht, Wt))]).astype(np.float32)
+    sc = S.Scene(H=48, W=52, means2D=means, radii=np.full(N, 4.0, np.float32),
+                 rotations=np.zeros(N, np.float32),
+                 opacities=rng.normal(size=N).astype(np.float32),
+                 colors=rng.normal(size=(N, 3)).astype(np.float32), templates=templates,
+                 bmp_sel=(np.arange(N) % T).astype(np.int32), c_blend=0.5,
+                 colors_orig_tmpl=rng.random((T, Ht, Wt, 3)).astype(np.float32))
+    sc = rounded(sc, dtype) if dtype != torch.float32 else sc
+    run = GpuRun(sc, dev, dtype=dtype, eps_T=0.0, co_mode=2, exact_blend=exact)
+    color, alpha = run.forward()
+    offs, idx = O.bin_scene(sc)
+    osc = O.scene_from(sc, offs, idx, blend_grad_exact=exact)
+    rc, ra, _ = O.forward(osc)
+    # the edge pixels are pushed: the last column / row of each primitive is drawn
+    assert ra[10, 14] > 0.1 and ra[14, 10] > 0.1
+    assert np.abs(color - rc).max() <= 1e-5 and np.abs(alpha - ra).max() <= 1e-5
+    gC, gA = rand_grads(sc.H, sc.W, seed=22)
+    g = run.backward(gC, gA)
+    err = normwise(g, O.backward(osc, gC.astype(np.float64), gA.astype(np.float64)))
+    assert max(err.values()) <= 1e-4, err
