@@ -31,7 +31,7 @@ def rounded(scene, dtype, fp32_geometry=False):
     return s
 
 
-class GpuRun:
+class DeviceRun:
     """Bins, renders and back-propagates one scene through the C ABI."""
 
     def __init__(self, scene, dev, dtype=torch.float32, eps_T=0.0, ts=16, co_mode=0,

@@ -51,9 +51,9 @@ def test_oracle_reproduces_fixture(path):
 @pytest.mark.parametrize("eps", [0.0, 1e-6])
 @pytest.mark.parametrize("path", FILES, ids=lambda p: os.path.basename(p)[:-4])
 def test_hip_matches_fixture(dev, path, eps):
-    from helpers import GpuRun, normwise
+    from helpers import DeviceRun, normwise
     sc, z = load(path)
-    run = GpuRun(sc, dev, eps_T=eps, co_mode=2 if sc.c_blend > 0 else 0)
+    run = DeviceRun(sc, dev, eps_T=eps, co_mode=2 if sc.c_blend > 0 else 0)
     assert np.array_equal(run.offs.cpu().numpy(), z["tile_offsets"])
     assert np.array_equal(run.idx.cpu().numpy(), z["tile_indices"])
     col, alp = run.forward()

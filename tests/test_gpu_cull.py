@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 from diffbmp_amd import scenes as S
-from helpers import GpuRun, normwise, rand_grads
+from helpers import DeviceRun, normwise, rand_grads
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +39,7 @@ def test_content_cull_exact(dev, seed):
     sc.rotations = rng.uniform(-np.pi, np.pi, N).astype(np.float32)
     sc.radii = np.exp(rng.uniform(np.log(3.0), np.log(40.0), N)).astype(np.float32)
     sc.means2D = np.stack([rng.uniform(-10, W + 10, N), rng.uniform(-10, H + 10, N)], 1).astype(np.float32)
-    run = GpuRun(sc, dev, eps_T=0.0)
+    run = DeviceRun(sc, dev, eps_T=0.0)
     col, alp = run.forward()
     osc = O.scene_from(sc, run.offs.cpu().numpy(), run.idx.cpu().numpy())
     rc, ra, _ = O.forward(osc)

@@ -13,7 +13,7 @@ from diffbmp_amd import _native
 from diffbmp_amd import scenes as S
 from diffbmp_amd.fit import FitJob
 from diffbmp_amd.tile_rasterizer import TileRasterizer
-from helpers import GpuRun, normwise, rand_grads
+from helpers import DeviceRun, normwise, rand_grads
 
 pytestmark = pytest.mark.gpu
 
@@ -62,7 +62,7 @@ def test_primitives_off_canvas(dev):
     off.means2D = sc.means2D.copy()
     far = np.arange(0, sc.N, 2)
     off.means2D[far] += np.array([4.0 * sc.W, -3.0 * sc.H], np.float32)
-    run = GpuRun(off, dev, eps_T=1e-6)
+    run = DeviceRun(off, dev, eps_T=1e-6)
     ext = O.template_extents_np(off.templates)
     offs, idx = O.bin_lists_c(off.means2D, off.radii, off.rotations, off.bmp_sel, ext, off.H, off.W,
                               off.tile_size)
@@ -80,7 +80,7 @@ def test_primitives_off_canvas(dev):
         assert np.abs(g[k][far]).max() == 0.0, k
     gone = copy.copy(sc)
     gone.means2D = sc.means2D + np.array([4.0 * sc.W, 0.0], np.float32)
-    run = GpuRun(gone, dev, eps_T=1e-6)
+    run = DeviceRun(gone, dev, eps_T=1e-6)
     assert run.idx.numel() == 0 and int(run.offs[-1].item()) == 0
     col, alp = run.forward()
     assert np.abs(col).max() == 0.0 and np.abs(alp).max() == 0.0
@@ -94,7 +94,7 @@ def test_canvas_smaller_than_a_block(dev, H, W):
     sc.means2D = (np.random.default_rng(8).uniform(-0.2, 1.2, (sc.N, 2)) *
                   np.array([W, H])).astype(np.float32)
     sc.radii = np.full(sc.N, max(H, W) / 3.0, np.float32)
-    run = GpuRun(sc, dev, eps_T=0.0)
+    run = DeviceRun(sc, dev, eps_T=0.0)
     ext = O.template_extents_np(sc.templates)
     offs, idx = O.bin_lists_c(sc.means2D, sc.radii, sc.rotations, sc.bmp_sel, ext, H, W, sc.tile_size)
     assert np.array_equal(run.offs.cpu().numpy(), offs) and np.array_equal(run.idx.cpu().numpy(), idx)
@@ -139,7 +139,7 @@ def test_colour_map_samples_on_the_last_texel(dev, dtype, exact):
                  bmp_sel=(np.arange(N) % T).astype(np.int32), c_blend=0.5,
                  colors_orig_tmpl=rng.random((T, Ht, Wt, 3)).astype(np.float32))
     sc = rounded(sc, dtype) if dtype != torch.float32 else sc
-    run = GpuRun(sc, dev, dtype=dtype, eps_T=0.0, co_mode=2, exact_blend=exact)
+    run = DeviceRun(sc, dev, dtype=dtype, eps_T=0.0, co_mode=2, exact_blend=exact)
     color, alpha = run.forward()
     offs, idx = O.bin_scene(sc)
     osc = O.scene_from(sc, offs, idx, blend_grad_exact=exact)

@@ -36,9 +36,9 @@ def test_loss_kernel_matches_autograd(dev):
 @pytest.mark.parametrize("with_bg", [False, True])
 def test_fused_backward_mse_matches_two_kernels(dev, with_bg):
     """dbmp_backward_mse equals dbmp_loss_mse followed by dbmp_backward."""
-    from helpers import GpuRun
+    from helpers import DeviceRun
     sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     run.forward()
     g = torch.Generator(dev).manual_seed(7)
     target = torch.rand((sc.H, sc.W, 3), device=dev, generator=g)
@@ -66,10 +66,10 @@ def test_backward_rows_accumulate_and_adam_clears_them(dev):
     """dbmp_backward_mse_rows adds the (N,8) gradient rows that
     dbmp_backward_mse unpacks; an Adam step with zero_grad reads the rows as
     the unpacked arrays and leaves them zero."""
-    from helpers import GpuRun
+    from helpers import DeviceRun
     from diffbmp_amd import _native
     sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     run.forward()
     target = torch.rand((sc.H, sc.W, 3), device=dev, generator=torch.Generator(dev).manual_seed(8))
     N = sc.N

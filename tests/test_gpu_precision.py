@@ -21,7 +21,7 @@ import torch
 
 import oracle as O
 from diffbmp_amd import scenes as S
-from helpers import GpuRun, rand_grads, rounded
+from helpers import DeviceRun, rand_grads, rounded
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +45,7 @@ def test_precision_error_table(dev, store):
     dt = {"fp16": torch.float16, "bf16": torch.bfloat16, "fp32": torch.float32}[store.split("-")[0]]
     mixed = store.endswith("-app")
     sc = S.make_scene(2, with_target=False)
-    run = GpuRun(sc, dev, dtype=dt, eps_T=1e-6, fp32_geometry=mixed)
+    run = DeviceRun(sc, dev, dtype=dt, eps_T=1e-6, fp32_geometry=mixed)
     col, alp = run.forward()
     rng = np.random.default_rng(17)
     tiles = np.sort(rng.choice(64 * 64, 16, replace=False)).astype(np.int32)

@@ -10,7 +10,7 @@ import torch
 import oracle as O
 from diffbmp_amd import _native
 from diffbmp_amd import scenes as S
-from helpers import GpuRun, normwise, rand_grads
+from helpers import DeviceRun, normwise, rand_grads
 
 pytestmark = pytest.mark.gpu
 
@@ -83,7 +83,7 @@ def test_push_masks_equal_culling(dev, kind, factor, eps):
         sc = _hard_ring_scene()
     else:
         sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=eps)
+    run = DeviceRun(sc, dev, eps_T=eps)
     gC, gA = rand_grads(sc.H, sc.W, seed=11)
     P = run.idx.numel() if factor == 1 else \
         run.h.bin(run.prims, run.ext, tuple(run.d["templates"].shape), factor)[1].numel()
@@ -109,7 +109,7 @@ def test_push_masks_equal_culling(dev, kind, factor, eps):
 def test_push_masks_blend_rgba(dev):
     """Colour-map blending (RGBA texel records, fp16 bitmaps), super-tile lists."""
     sc = S.make_scene(3, H=192, W=160, N=300, tsz=64, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6, co_mode=2, bm_dtype=torch.float16)
+    run = DeviceRun(sc, dev, eps_T=1e-6, co_mode=2, bm_dtype=torch.float16)
     gC, gA = rand_grads(sc.H, sc.W, seed=12)
     P = run.h.bin(run.prims, run.ext, tuple(run.d["templates"].shape), 4)[1].numel()
     L = _bin(run, 4, P + 64)
@@ -121,7 +121,7 @@ def test_push_masks_unused_after_overflow(dev):
     """A pair overflow walks every primitive: the forward writes no masks and
     the backward culls, with the same results as complete lists."""
     sc = _hard_ring_scene()
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     gC, gA = rand_grads(sc.H, sc.W, seed=13)
     P = run.h.bin(run.prims, run.ext, tuple(run.d["templates"].shape), 4)[1].numel()
     full = _bin(run, 4, P + 64)
@@ -137,7 +137,7 @@ def test_push_masks_need_a_capacity(dev):
     """Lists without a pair count (a caller's complete mapping) carry no
     capacity to size the masks by: the mask buffer is ignored."""
     sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     gC, gA = rand_grads(sc.H, sc.W, seed=14)
     pm = torch.full((1 << 16,), -1, dtype=torch.int64, device=dev)
     L = _native.Lists(run.offs, run.idx, masks=pm)

@@ -5,7 +5,7 @@ import torch
 
 import oracle as O
 from diffbmp_amd import scenes as S
-from helpers import GpuRun, normwise, rand_grads, rounded
+from helpers import DeviceRun, normwise, rand_grads, rounded
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,7 @@ def _oracle_lists(scene):
 @pytest.mark.parametrize("cfg", [0, 1, 3])
 def test_binning_bit_exact(dev, cfg):
     sc = S.make_scene(cfg, with_target=False)
-    run = GpuRun(sc, dev)
+    run = DeviceRun(sc, dev)
     ext_ref = O.template_extents_np(sc.templates)
     assert np.array_equal(run.ext.cpu().numpy(), ext_ref)
     offs, idx = _oracle_lists(sc)
@@ -32,7 +32,7 @@ def test_binning_bit_exact(dev, cfg):
 @pytest.mark.parametrize("eps", [0.0, 1e-6])
 def test_forward_plain(dev, eps):
     sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=eps)
+    run = DeviceRun(sc, dev, eps_T=eps)
     col, alp = run.forward()
     osc = O.scene_from(sc, run.offs.cpu().numpy(), run.idx.cpu().numpy())
     rc, ra, _ = O.forward(osc)
@@ -44,7 +44,7 @@ def test_forward_plain(dev, eps):
 @pytest.mark.parametrize("eps", [0.0, 1e-6])
 def test_backward_plain(dev, eps):
     sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=eps)
+    run = DeviceRun(sc, dev, eps_T=eps)
     run.forward()
     gC, gA = rand_grads(sc.H, sc.W, seed=3)
     g = run.backward(gC, gA)
@@ -58,7 +58,7 @@ def test_backward_plain(dev, eps):
 def test_scale_sampled_tiles(dev):
     """Full 1024^2 / 8192-primitive workload; oracle on a sample of tiles."""
     sc = S.make_scene(1, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     col, alp = run.forward()
     rng = np.random.default_rng(7)
     tiles = np.sort(rng.choice(64 * 64, 24, replace=False)).astype(np.int32)
@@ -102,7 +102,7 @@ def test_overflow_fallback_is_exact(dev, kind, ts, explicit):
     from diffbmp_amd import _native
     sc = _hard_scene() if kind == "hard" else S.make_scene(0, with_target=False)
     sc.tile_size = ts
-    run = GpuRun(sc, dev, eps_T=1e-6, ts=ts)
+    run = DeviceRun(sc, dev, eps_T=1e-6, ts=ts)
     col, alp = run.forward()
     gC, gA = rand_grads(sc.H, sc.W, seed=9)
     g = run.backward(gC, gA)
@@ -148,7 +148,7 @@ def test_lazy_binning_overflow_then_grows(dev, ts):
     from diffbmp_amd import _native
     sc = _hard_scene()
     sc.tile_size = ts
-    run = GpuRun(sc, dev, eps_T=1e-6, ts=ts)
+    run = DeviceRun(sc, dev, eps_T=1e-6, ts=ts)
     col, alp = run.forward()
     h, shape = run.h, tuple(run.d["templates"].shape)
     P = h.bin(run.prims, run.ext, shape, h.bin_factor)[1].numel()   # pairs of the lists bin_lazy builds
@@ -174,7 +174,7 @@ def test_lazy_binning_overflow_then_grows(dev, ts):
 def test_ragged_canvas_and_tile_sizes(dev, H, W, ts):
     sc = S.make_scene(0, H=H, W=W, N=40, tsz=32, with_target=False)
     sc.tile_size = ts
-    run = GpuRun(sc, dev, eps_T=0.0, ts=ts)
+    run = DeviceRun(sc, dev, eps_T=0.0, ts=ts)
     offs, idx = _oracle_lists(sc)
     assert np.array_equal(run.offs.cpu().numpy(), offs) and np.array_equal(run.idx.cpu().numpy(), idx)
     col, alp = run.forward()
@@ -196,7 +196,7 @@ def test_empty_and_degenerate_scenes(dev):
     sc.bmp_sel = sc.bmp_sel.copy()
     sc.bmp_sel[5] = 3             # only one template: invalid, never drawn
     sc.radii[6] = 1e-7
-    run = GpuRun(sc, dev, eps_T=0.0)
+    run = DeviceRun(sc, dev, eps_T=0.0)
     col, alp = run.forward()
     offs, idx = _oracle_lists(sc)
     assert np.array_equal(run.offs.cpu().numpy(), offs) and np.array_equal(run.idx.cpu().numpy(), idx)
@@ -211,7 +211,7 @@ def test_empty_and_degenerate_scenes(dev):
     empty = copy.copy(sc)
     for k in ("means2D", "radii", "rotations", "opacities", "colors", "bmp_sel"):
         setattr(empty, k, getattr(sc, k)[:0])
-    run0 = GpuRun(empty, dev)
+    run0 = DeviceRun(empty, dev)
     col0, alp0 = run0.forward()
     assert run0.idx.numel() == 0 and col0.max() == 0 and alp0.max() == 0
 
@@ -220,7 +220,7 @@ def test_repeat_forward_is_bit_identical(dev):
     """The second forward of a handle runs its workgroups heaviest first (the
     order measured by the first); every pixel's result is unchanged."""
     sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     c1, a1 = run.forward()
     T1, l1 = run.T.cpu().numpy(), run.last.cpu().numpy()
     gC, gA = rand_grads(sc.H, sc.W, seed=3)

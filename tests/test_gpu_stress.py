@@ -16,7 +16,7 @@ import torch
 import oracle as O
 from diffbmp_amd import scenes as S
 from diffbmp_amd._native import Handle
-from helpers import GpuRun, fd_scalars, fd_window, half_rounded, normwise, rand_grads
+from helpers import DeviceRun, fd_scalars, fd_window, half_rounded, normwise, rand_grads
 
 pytestmark = pytest.mark.gpu
 
@@ -32,7 +32,7 @@ def _stress_scene(window=False):
 
 
 def _gpu(sc, dev, eps, exact):
-    run = GpuRun(sc, dev, eps_T=eps, co_mode=2, bm_dtype=torch.float16)
+    run = DeviceRun(sc, dev, eps_T=eps, co_mode=2, bm_dtype=torch.float16)
     run.bm = Handle.bitmaps(run.d["templates"], run.co, 2, sc.c_blend, -1, exact)
     return run
 

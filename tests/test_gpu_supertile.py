@@ -8,7 +8,7 @@ import torch
 import oracle as O
 from diffbmp_amd import _native
 from diffbmp_amd import scenes as S
-from helpers import GpuRun, rand_grads
+from helpers import DeviceRun, rand_grads
 
 pytestmark = pytest.mark.gpu
 
@@ -49,7 +49,7 @@ def _check_same(a, b):
 def test_supertile_lists_match_per_tile(dev, cfg, H, W, factor):
     kw = {} if H is None else dict(H=H, W=W, N=120, tsz=32)
     sc = S.make_scene(cfg, with_target=False, **kw)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     gC, gA = rand_grads(sc.H, sc.W, seed=5)
     ref = _raster(run, _native.Lists(run.offs, run.idx), gC, gA)
     offs, idx, rg = run.h.bin(run.prims, run.ext, tuple(run.d["templates"].shape), factor)
@@ -66,7 +66,7 @@ def test_supertile_lists_match_per_tile(dev, cfg, H, W, factor):
 def test_supertile_blend_rgba(dev):
     """Colour-map blending with RGBA texel records (fp16 bitmaps)."""
     sc = S.make_scene(3, H=192, W=160, N=300, tsz=64, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6, co_mode=2, bm_dtype=torch.float16)
+    run = DeviceRun(sc, dev, eps_T=1e-6, co_mode=2, bm_dtype=torch.float16)
     gC, gA = rand_grads(sc.H, sc.W, seed=6)
     ref = _raster(run, _native.Lists(run.offs, run.idx), gC, gA)
     offs, idx, rg = run.h.bin(run.prims, run.ext, tuple(run.d["templates"].shape), 4)
@@ -77,7 +77,7 @@ def test_supertile_overflow_fallback(dev):
     """Super-tile binning into a too-small buffer: the raster kernels walk
     every primitive, filtered by the tile ranges, with the same results."""
     sc = S.make_scene(0, with_target=False)
-    run = GpuRun(sc, dev, eps_T=1e-6)
+    run = DeviceRun(sc, dev, eps_T=1e-6)
     gC, gA = rand_grads(sc.H, sc.W, seed=7)
     ref = _raster(run, _native.Lists(run.offs, run.idx), gC, gA)
     _, full, _ = run.h.bin(run.prims, run.ext, tuple(run.d["templates"].shape), 4)
@@ -94,7 +94,7 @@ def test_supertile_overflow_fallback(dev):
 
 def test_supertile_argument_checks(dev):
     sc = S.make_scene(0, H=64, W=64, N=10, tsz=32, with_target=False)
-    run = GpuRun(sc, dev)
+    run = DeviceRun(sc, dev)
     shape = tuple(run.d["templates"].shape)
     offs = torch.empty(64, dtype=torch.int32, device=dev)
     idx = torch.empty(1024, dtype=torch.int32, device=dev)

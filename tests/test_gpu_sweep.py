@@ -30,7 +30,7 @@ import torch
 import oracle as O
 from diffbmp_amd import scenes as S
 from diffbmp_amd import tile_rasterizer as TR
-from helpers import GpuRun, normwise, rand_grads, rounded
+from helpers import DeviceRun, normwise, rand_grads, rounded
 
 pytestmark = pytest.mark.gpu
 
@@ -89,7 +89,7 @@ def _dropin(sc, opts, dev, gC, gA):
 
 
 def _capi(sc, opts, dev, gC, gA):
-    run = GpuRun(sc, dev, dtype=torch.float16, eps_T=opts["eps"], ts=sc.tile_size,
+    run = DeviceRun(sc, dev, dtype=torch.float16, eps_T=opts["eps"], ts=sc.tile_size,
                  co_mode=opts["co_mode"], exact_blend=opts["exact"],
                  fp32_geometry=opts["storage"] == "mixed")
     color, alpha = run.forward()
