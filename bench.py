@@ -250,13 +250,15 @@ def run_fused(args, sc, dev, sdt):
     return job, job.step, job.handle, fwd_bwd, reset
 
 
-def run_autograd(args, sc, dev, sdt, fused_loss=False, calling_thread=False):
+def run_autograd(args, sc, dev, sdt, fused_loss=False, calling_thread=False, fused_adam=False):
     """The reference-compatible TileRasterizer behind torch autograd, with the
     background composite + MSE in torch as the reference's renderer writes it,
     or (`fused_loss`) through the one-kernel composite_mse_loss op.
     `calling_thread`: the backward runs on the thread that calls it
     (torch.autograd.set_multithreading_enabled(False)) instead of being handed
-    to torch's per-device autograd thread."""
+    to torch's per-device autograd thread.  `fused_adam`: the step's Adam and
+    clamp are diffbmp_amd.optim.FusedAdam (one kernel) instead of
+    torch.optim.Adam and three clamp_ calls."""
     import torch
     import torch.nn.functional as F
     from diffbmp_amd.tile_rasterizer import TileRasterizer
@@ -266,11 +268,17 @@ def run_autograd(args, sc, dev, sdt, fused_loss=False, calling_thread=False):
     rast = TileRasterizer(H, W, 16, 0.0, 1.0, 151, N, use_fp16=(sdt != torch.float32), storage_dtype=sdt)
     if args.eps_t is not None:
         rast.eps_T = args.eps_t
-    opt = torch.optim.Adam(leaves, lr=1e-2)
+    rmax = float(min(H, W) // 4)
+    if fused_adam:
+        from diffbmp_amd.optim import FusedAdam
+        opt = FusedAdam([{"params": [leaves[0]], "clamp": ((0.0, 0.0), (float(W - 1), float(H - 1)))},
+                         {"params": [leaves[1]], "clamp": (2.0, rmax)},
+                         {"params": leaves[2:]}], lr=1e-2, handle=rast.handle)
+    else:
+        opt = torch.optim.Adam(leaves, lr=1e-2)
     bg = torch.ones((H, W, 3), device=dev)
     empty = torch.empty(0, device=dev)
     lr_conf = torch.tensor([1e-2, 1, 1, 1, 1, 1, 1], device=dev)
-    rmax = float(min(H, W) // 4)
     templates, sel, target = pr["templates"], pr["sel"], pr["target"]
 
     def fwd_bwd():
@@ -289,10 +297,11 @@ def run_autograd(args, sc, dev, sdt, fused_loss=False, calling_thread=False):
     def step():
         fwd_bwd()
         opt.step()
-        with torch.no_grad():
-            leaves[0][:, 0].clamp_(0.0, float(W - 1))
-            leaves[0][:, 1].clamp_(0.0, float(H - 1))
-            leaves[1].clamp_(2.0, rmax)
+        if not fused_adam:
+            with torch.no_grad():
+                leaves[0][:, 0].clamp_(0.0, float(W - 1))
+                leaves[0][:, 1].clamp_(0.0, float(H - 1))
+                leaves[1].clamp_(2.0, rmax)
 
     return rast, step, rast.handle, fwd_bwd, (lambda: None)
 
@@ -749,6 +758,8 @@ def main():
     res_fl = time_mode(args, lambda a, s, d, t: run_autograd(a, s, d, t, fused_loss=True), sc, dev, sdt)
     res_ct = time_mode(args, lambda a, s, d, t: run_autograd(a, s, d, t, fused_loss=True,
                                                              calling_thread=True), sc, dev, sdt)
+    res_fa = time_mode(args, lambda a, s, d, t: run_autograd(a, s, d, t, fused_loss=True,
+                                                             fused_adam=True), sc, dev, sdt)
     eps_T = args.eps_t if args.eps_t is not None else 1e-6
     P, list_mean, walked = list_stats(res["handle"], sc, dev, sdt, eps_T)
 
@@ -802,6 +813,9 @@ def main():
         # host time that this launch-heavy pass cannot hide
         "autograd_fused_loss_calling_thread_iters_per_s": round(args.steps / res_ct["total"], 3),
         "autograd_fused_loss_calling_thread_fwd_bwd_iters_per_s": round(1.0 / res_ct["fwd_bwd_s"], 3),
+        # the drop-in step with diffbmp_amd.optim.FusedAdam (Adam + clamp in
+        # one kernel) in place of torch.optim.Adam and the clamp_ calls
+        "autograd_fused_loss_fused_adam_iters_per_s": round(args.steps / res_fa["total"], 3),
         "samples_per_s_fwd": float(f"{walked * H * W / res['fwd_s']:.4g}"),
         "pairs": P,
         "list_len_mean": round(list_mean, 2),

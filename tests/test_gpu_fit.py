@@ -323,3 +323,50 @@ def test_composite_mse_loss_repeated_backward(dev):
         torch.cuda.synchronize()
         for x, y in ((c2, c1), (a2, a1)):
             assert torch.allclose(x.grad, y.grad, rtol=1e-5, atol=1e-9), (k, s)
+
+
+def test_fused_adam_matches_torch_adam(dev):
+    """FusedAdam against torch.optim.Adam followed by the reference's clamp:
+    six tensors (two launches per step) in parameter groups with different
+    lr, per-column and scalar clamps, fresh random gradients each step, 6
+    steps; then a tensor added later, whose step count differs."""
+    from diffbmp_amd.optim import FusedAdam
+    g = torch.Generator(dev).manual_seed(3)
+    shapes = [(300, 2), (300,), (300,), (300,), (300, 3), (77,)]
+    init = [torch.rand(s, device=dev, generator=g) * 20 for s in shapes]
+    a = [x.clone().requires_grad_() for x in init]
+    b = [x.clone().requires_grad_() for x in init]
+    lo, hi = (1.0, 2.0), (15.0, 12.0)
+    ref = torch.optim.Adam([{"params": a[:2], "lr": 0.5}, {"params": a[2:], "lr": 0.1}],
+                           betas=(0.9, 0.99), eps=1e-8)
+    fused = FusedAdam([{"params": [b[0]], "lr": 0.5, "clamp": (lo, hi)},
+                       {"params": [b[1]], "lr": 0.5, "clamp": (2.0, 10.0)},
+                       {"params": b[2:], "lr": 0.1}], betas=(0.9, 0.99), eps=1e-8)
+    for _ in range(6):
+        grads = [torch.randn(s, device=dev, generator=g) for s in shapes]
+        for x, y, gr in zip(a, b, grads):
+            x.grad, y.grad = gr.clone(), gr.clone()
+        ref.step()
+        with torch.no_grad():
+            a[0][:, 0].clamp_(lo[0], hi[0])
+            a[0][:, 1].clamp_(lo[1], hi[1])
+            a[1].clamp_(2.0, 10.0)
+        fused.step()
+        for x, y in zip(a, b):
+            assert torch.allclose(x, y, rtol=1e-5, atol=1e-5), (x - y).abs().max()
+    # a tensor added later: its own step count, its own launch
+    c = torch.rand(100, device=dev, generator=g).requires_grad_()
+    c2 = c.detach().clone().requires_grad_()
+    fused.add_param_group({"params": [c], "lr": 0.2})
+    ref.add_param_group({"params": [c2], "lr": 0.2})
+    for x, y in zip(a + [c2], b + [c]):
+        gr = torch.randn(x.shape, device=dev, generator=g)
+        x.grad, y.grad = gr.clone(), gr.clone()
+    ref.step()
+    with torch.no_grad():
+        a[0][:, 0].clamp_(lo[0], hi[0])
+        a[0][:, 1].clamp_(lo[1], hi[1])
+        a[1].clamp_(2.0, 10.0)
+    fused.step()
+    for x, y in zip(a + [c2], b + [c]):
+        assert torch.allclose(x, y, rtol=1e-5, atol=1e-5), (x - y).abs().max()
