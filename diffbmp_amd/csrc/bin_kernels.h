@@ -14,9 +14,9 @@
 //                 primitive ids ascending inside each tile.
 //   k_tile_bounds CSR offsets from the boundaries of the sorted tile ids.
 // For few lists x primitives, the lists are built directly instead:
-//   k_tile_count  one workgroup per list: its primitives, counted per quarter
-//                 of the primitive range (one quarter per wave).
-//   k_tile_fill   one workgroup per list: its offset from the counts, then each
+//   k_tile_build  one workgroup per list: its primitives counted per quarter
+//                 of the primitive range (one quarter per wave), its offset in
+//                 closed form from every primitive's tile range, then each
 //                 wave appends its quarter's primitives in id order (ballot
 //                 compaction), so the lists equal the sorted ones.
 #pragma once
@@ -528,14 +528,41 @@ __device__ __forceinline__ bool list_covers(int4 r, int tx, int ty, int shift) {
          (ty <= (r.w >> shift));
 }
 
-static __global__ __launch_bounds__(256) void k_tile_count(const int4* __restrict__ rg, int N,
-                                                           int tilesX, int shift,
-                                                           int* __restrict__ counts4) {
+// Pairs of the lists before list (tx, ty) (row-major, tilesX per row) and in
+// all lists, of a primitive whose lists are [x0, x1] x [y0, y1] (empty when
+// x1 < x0 or y1 < y0).
+__device__ __forceinline__ void pairs_before(int4 r, int tx, int ty, int tilesX, int shift,
+                                             long long& before, long long& all) {
+  const int x0 = r.x >> shift, x1 = r.y >> shift, y0 = r.z >> shift, y1 = r.w >> shift;
+  const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
+  if (nx <= 0 || ny <= 0) return;
+  all += (long long)nx * ny;
+  const int rows = max(0, min(y1, ty - 1) - y0 + 1);   // whole rows above ty
+  const int cols = (y0 <= ty && ty <= y1) ? max(0, min(x1, tx - 1) - x0 + 1) : 0;
+  before += (long long)rows * nx + cols;
+  (void)tilesX;
+}
+
+// One workgroup per list.  First pass: each wave counts its quarter's
+// primitives in this list and, in closed form per primitive, the pairs of
+// all lists before this one and of all lists; second pass: each wave appends
+// its quarter's primitives in id order (ballot compaction), so the lists
+// equal the sorted ones.  offs (tiles+1): list offsets; total: pair count.
+// Lists are written only when the pairs fit cap (otherwise every offset is 0
+// and the raster kernels walk all primitives).
+static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restrict__ rg, int N,
+                                                           int tilesX, int tiles, int shift,
+                                                           long long cap, int* __restrict__ offs,
+                                                           int* __restrict__ idx,
+                                                           long long* __restrict__ total) {
+  __shared__ long long sred[2][4];
+  __shared__ int scnt[4];
   const int t = blockIdx.x;
   const int tx = t % tilesX, ty = t / tilesX;
   const int w = threadIdx.x >> 6, l = lane_id();
   const int q = (N + 3) / 4;
   const int lo = w * q, hi = min(N, lo + q);
+  long long before = 0, all = 0;
   int c = 0;
   for (int base = lo; base < hi; base += 64 * kDirectBatch) {
     int4 r[kDirectBatch];
@@ -545,42 +572,22 @@ static __global__ __launch_bounds__(256) void k_tile_count(const int4* __restric
       r[k] = n < hi ? rg[n] : make_int4(0, -1, 0, -1);
     }
 #pragma unroll
-    for (int k = 0; k < kDirectBatch; ++k) c += list_covers(r[k], tx, ty, shift) ? 1 : 0;
+    for (int k = 0; k < kDirectBatch; ++k) {
+      c += list_covers(r[k], tx, ty, shift) ? 1 : 0;
+      pairs_before(r[k], tx, ty, tilesX, shift, before, all);
+    }
   }
   c = wave_sum_i32(c);
-  if (l == 0) counts4[4 * t + w] = c;
-}
-
-// offs (tiles+1): list offsets; total: pair count.  Lists are written only
-// when the pairs fit cap (otherwise the raster kernels walk all primitives).
-static __global__ __launch_bounds__(256) void k_tile_fill(const int4* __restrict__ rg, int N,
-                                                          int tilesX, int tiles, int shift,
-                                                          const int* __restrict__ counts4,
-                                                          long long cap, int* __restrict__ offs,
-                                                          int* __restrict__ idx,
-                                                          long long* __restrict__ total) {
-  __shared__ long long sred[2][4];
-  const int t = blockIdx.x;
-  const int tx = t % tilesX, ty = t / tilesX;
-  const int w = threadIdx.x >> 6, l = lane_id();
-  // pairs of all lists before this one, and of all lists
-  long long before = 0, all = 0;
-  for (int i = threadIdx.x; i < 4 * tiles; i += 256) {
-    const long long c = counts4[i];
-    all += c;
-    before += (i < 4 * t) ? c : 0;
-  }
   before = wave_sum_i64(before);
   all = wave_sum_i64(all);
   if (l == 0) {
     sred[0][w] = before;
     sred[1][w] = all;
+    scnt[w] = c;
   }
   __syncthreads();
   before = sred[0][0] + sred[0][1] + sred[0][2] + sred[0][3];
   all = sred[1][0] + sred[1][1] + sred[1][2] + sred[1][3];
-  // on overflow (all > cap) every offset is 0, as k_tile_bounds leaves them,
-  // and the raster kernels walk every primitive
   const bool fits = all <= cap;
   if (threadIdx.x == 0) {
     offs[t] = fits ? (int)before : 0;
@@ -591,9 +598,7 @@ static __global__ __launch_bounds__(256) void k_tile_fill(const int4* __restrict
   }
   if (!fits) return;
   long long pos = before;
-  for (int k = 0; k < w; ++k) pos += counts4[4 * t + k];
-  const int q = (N + 3) / 4;
-  const int lo = w * q, hi = min(N, lo + q);
+  for (int k = 0; k < w; ++k) pos += scnt[k];
   const unsigned long long lt = (1ull << l) - 1ull;
   for (int base = lo; base < hi; base += 64 * kDirectBatch) {
     int4 r[kDirectBatch];

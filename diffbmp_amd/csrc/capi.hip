@@ -113,8 +113,6 @@ struct dbmp_rasterizer {
   int hist_blocks = 0;
   int* hist = nullptr;
   int* totals = nullptr;
-  int tcount_cap = 0;
-  int* tcount4 = nullptr;     // direct list build: (lists, 4) counts
   // per-block forward costs and the heaviest-first block order derived from
   // them (valid for order_n blocks; 0: none yet)
   int blk_cap = 0;
@@ -493,7 +491,7 @@ int dbmp_destroy(dbmp_rasterizer* h) {
   timing_drain(h->ev_bwd, ms, n);
   void* ptrs[] = {h->recs, h->counts, h->offs, h->ranges, h->grad8, h->keysA,
                   h->keysB, h->valsB, h->hist, h->totals, h->quads, h->blk_cost, h->blk_order,
-                  h->tcount4, h->cull};
+                  h->cull};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -520,7 +518,7 @@ int dbmp_template_extents(dbmp_rasterizer* h, const dbmp_bitmaps* bm, float* ext
   return 0;
 }
 
-// Lists built directly (k_tile_count / k_tile_fill) when there are at most
+// Lists built directly (k_tile_build) when there are at most
 // this many lists and list x primitive tests; a radix sort otherwise.
 constexpr int kDirectMaxLists = 1024;
 constexpr long long kDirectMaxTests = 1LL << 23;
@@ -553,16 +551,9 @@ int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext
   if ((rc = launch_prep(h, prims, ext, num_templates, tmpl_h, tmpl_w, st, rg, shift))) return rc;
   if (tiles <= kDirectMaxLists && (long long)tiles * N <= kDirectMaxTests) {
     // few lists x primitives: build each list directly from the ranges
-    if (tiles > h->tcount_cap) {
-      if ((rc = grow(&h->tcount4, 4LL * tiles))) return rc;
-      h->tcount_cap = tiles;
-    }
-    hipLaunchKernelGGL(k_tile_count, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX, shift,
-                       h->tcount4);
-    LAUNCH_CHECK("k_tile_count");
-    hipLaunchKernelGGL(k_tile_fill, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX, tiles,
-                       shift, h->tcount4, (long long)capacity, tile_offsets, tile_indices, dtot);
-    LAUNCH_CHECK("k_tile_fill");
+    hipLaunchKernelGGL(k_tile_build, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX, tiles,
+                       shift, (long long)capacity, tile_offsets, tile_indices, dtot);
+    LAUNCH_CHECK("k_tile_build");
     return 0;
   }
   scan_counts(h->counts, N, h->offs, dtot, h->offs + h->n_cap + 1, st);
