@@ -531,8 +531,8 @@ __device__ __forceinline__ bool list_covers(int4 r, int tx, int ty, int shift) {
 // Pairs of the lists before list (tx, ty) (row-major, tilesX per row) and in
 // all lists, of a primitive whose lists are [x0, x1] x [y0, y1] (empty when
 // x1 < x0 or y1 < y0).
-__device__ __forceinline__ void pairs_before(int4 r, int tx, int ty, int tilesX, int shift,
-                                             long long& before, long long& all) {
+__device__ __forceinline__ void pairs_before(int4 r, int tx, int ty, int shift, long long& before,
+                                             long long& all) {
   const int x0 = r.x >> shift, x1 = r.y >> shift, y0 = r.z >> shift, y1 = r.w >> shift;
   const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
   if (nx <= 0 || ny <= 0) return;
@@ -540,7 +540,6 @@ __device__ __forceinline__ void pairs_before(int4 r, int tx, int ty, int tilesX,
   const int rows = max(0, min(y1, ty - 1) - y0 + 1);   // whole rows above ty
   const int cols = (y0 <= ty && ty <= y1) ? max(0, min(x1, tx - 1) - x0 + 1) : 0;
   before += (long long)rows * nx + cols;
-  (void)tilesX;
 }
 
 // One workgroup per list.  First pass: each wave counts its quarter's
@@ -574,7 +573,7 @@ static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restric
 #pragma unroll
     for (int k = 0; k < kDirectBatch; ++k) {
       c += list_covers(r[k], tx, ty, shift) ? 1 : 0;
-      pairs_before(r[k], tx, ty, tilesX, shift, before, all);
+      pairs_before(r[k], tx, ty, shift, before, all);
     }
   }
   c = wave_sum_i32(c);
