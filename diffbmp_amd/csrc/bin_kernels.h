@@ -549,6 +549,10 @@ __device__ __forceinline__ void pairs_before(int4 r, int tx, int ty, int shift, 
 // equal the sorted ones.  offs (tiles+1): list offsets; total: pair count.
 // Lists are written only when the pairs fit cap (otherwise every offset is 0
 // and the raster kernels walk all primitives).
+// STAGE (N <= kDirectStage): the first pass also keeps the ranges in LDS for
+// the second (each wave reads back only what it wrote).
+constexpr int kDirectStage = 8192;
+template <bool STAGE>
 static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restrict__ rg, int N,
                                                            int tilesX, int tiles, int shift,
                                                            long long cap, int* __restrict__ offs,
@@ -556,6 +560,7 @@ static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restric
                                                            long long* __restrict__ total) {
   __shared__ long long sred[2][4];
   __shared__ int scnt[4];
+  __shared__ int4 srg[STAGE ? kDirectStage : 1];
   const int t = blockIdx.x;
   const int tx = t % tilesX, ty = t / tilesX;
   const int w = threadIdx.x >> 6, l = lane_id();
@@ -574,6 +579,7 @@ static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restric
     for (int k = 0; k < kDirectBatch; ++k) {
       c += list_covers(r[k], tx, ty, shift) ? 1 : 0;
       pairs_before(r[k], tx, ty, shift, before, all);
+      if (STAGE && base + k * 64 + l < hi) srg[base + k * 64 + l] = r[k];
     }
   }
   c = wave_sum_i32(c);
@@ -604,7 +610,7 @@ static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restric
 #pragma unroll
     for (int k = 0; k < kDirectBatch; ++k) {
       const int n = base + k * 64 + l;
-      r[k] = n < hi ? rg[n] : make_int4(0, -1, 0, -1);
+      r[k] = n < hi ? (STAGE ? srg[n] : rg[n]) : make_int4(0, -1, 0, -1);
     }
 #pragma unroll
     for (int k = 0; k < kDirectBatch; ++k) {

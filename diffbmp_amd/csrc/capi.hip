@@ -551,8 +551,12 @@ int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext
   if ((rc = launch_prep(h, prims, ext, num_templates, tmpl_h, tmpl_w, st, rg, shift))) return rc;
   if (tiles <= kDirectMaxLists && (long long)tiles * N <= kDirectMaxTests) {
     // few lists x primitives: build each list directly from the ranges
-    hipLaunchKernelGGL(k_tile_build, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX, tiles,
-                       shift, (long long)capacity, tile_offsets, tile_indices, dtot);
+    if (N <= kDirectStage)
+      hipLaunchKernelGGL(k_tile_build<true>, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX,
+                         tiles, shift, (long long)capacity, tile_offsets, tile_indices, dtot);
+    else
+      hipLaunchKernelGGL(k_tile_build<false>, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX,
+                         tiles, shift, (long long)capacity, tile_offsets, tile_indices, dtot);
     LAUNCH_CHECK("k_tile_build");
     return 0;
   }
