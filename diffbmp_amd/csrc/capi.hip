@@ -189,7 +189,7 @@ int ensure_blocks(dbmp_rasterizer* h, int nb) {
 // L2.  Larger bitmaps are L2-bound: there the natural order, which runs
 // neighbouring tiles (sharing texels) together, is faster.
 constexpr size_t kOrderMaxQuadBytes = 4u << 20;
-constexpr int kOrderEvery = 4;
+constexpr int kOrderEvery = 8;
 bool uses_block_order(const dbmp_rasterizer* h) { return h->quads_used <= kOrderMaxQuadBytes; }
 const int* block_order(const dbmp_rasterizer* h, int nb) {
   return (h->order_n == nb && uses_block_order(h)) ? h->blk_order : nullptr;
