@@ -14,10 +14,10 @@
 //                 primitive ids ascending inside each tile.
 //   k_tile_bounds CSR offsets from the boundaries of the sorted tile ids.
 // For few lists x primitives, the lists are built directly instead:
-//   k_tile_build  one workgroup per list: its primitives counted per quarter
-//                 of the primitive range (one quarter per wave), its offset in
+//   k_tile_build  one workgroup per list: its primitives counted per
+//                 sixteenth of the primitive range (one per wave), its offset in
 //                 closed form from every primitive's tile range, then each
-//                 wave appends its quarter's primitives in id order (ballot
+//                 wave appends its share's primitives in id order (ballot
 //                 compaction), so the lists equal the sorted ones.
 #pragma once
 #include "dbmp_device.h"
@@ -520,8 +520,9 @@ static __global__ __launch_bounds__(256) void k_tile_bounds(const int* __restric
 // Direct list build.  A primitive n belongs to list t = (tx, ty) (lists of
 // 2^shift x 2^shift tiles, tilesX lists per row) iff its tile range rg[n]
 // covers it.  Wave w of a workgroup owns primitives [w*q, (w+1)*q), q =
-// ceil(N/4); eight range loads per lane are in flight at a time.
+// ceil(N/kBuildWaves); eight range loads per lane are in flight at a time.
 constexpr int kDirectBatch = 8;
+constexpr int kBuildWaves = 16;   // waves per list workgroup (1024 threads)
 
 __device__ __forceinline__ bool list_covers(int4 r, int tx, int ty, int shift) {
   return ((r.x >> shift) <= tx) & (tx <= (r.y >> shift)) & ((r.z >> shift) <= ty) &
@@ -542,10 +543,10 @@ __device__ __forceinline__ void pairs_before(int4 r, int tx, int ty, int shift, 
   before += (long long)rows * nx + cols;
 }
 
-// One workgroup per list.  First pass: each wave counts its quarter's
+// One workgroup per list.  First pass: each wave counts its share's
 // primitives in this list and, in closed form per primitive, the pairs of
 // all lists before this one and of all lists; second pass: each wave appends
-// its quarter's primitives in id order (ballot compaction), so the lists
+// its share's primitives in id order (ballot compaction), so the lists
 // equal the sorted ones.  offs (tiles+1): list offsets; total: pair count.
 // Lists are written only when the pairs fit cap (otherwise every offset is 0
 // and the raster kernels walk all primitives).
@@ -553,18 +554,18 @@ __device__ __forceinline__ void pairs_before(int4 r, int tx, int ty, int shift, 
 // the second (each wave reads back only what it wrote).
 constexpr int kDirectStage = 8192;
 template <bool STAGE>
-static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restrict__ rg, int N,
+static __global__ __launch_bounds__(64 * kBuildWaves) void k_tile_build(const int4* __restrict__ rg, int N,
                                                            int tilesX, int tiles, int shift,
                                                            long long cap, int* __restrict__ offs,
                                                            int* __restrict__ idx,
                                                            long long* __restrict__ total) {
-  __shared__ long long sred[2][4];
-  __shared__ int scnt[4];
+  __shared__ long long sred[2][kBuildWaves];
+  __shared__ int scnt[kBuildWaves];
   __shared__ int4 srg[STAGE ? kDirectStage : 1];
   const int t = blockIdx.x;
   const int tx = t % tilesX, ty = t / tilesX;
   const int w = threadIdx.x >> 6, l = lane_id();
-  const int q = (N + 3) / 4;
+  const int q = (N + kBuildWaves - 1) / kBuildWaves;
   const int lo = w * q, hi = min(N, lo + q);
   long long before = 0, all = 0;
   int c = 0;
@@ -591,8 +592,13 @@ static __global__ __launch_bounds__(256) void k_tile_build(const int4* __restric
     scnt[w] = c;
   }
   __syncthreads();
-  before = sred[0][0] + sred[0][1] + sred[0][2] + sred[0][3];
-  all = sred[1][0] + sred[1][1] + sred[1][2] + sred[1][3];
+  before = 0;
+  all = 0;
+#pragma unroll
+  for (int k = 0; k < kBuildWaves; ++k) {
+    before += sred[0][k];
+    all += sred[1][k];
+  }
   const bool fits = all <= cap;
   if (threadIdx.x == 0) {
     offs[t] = fits ? (int)before : 0;

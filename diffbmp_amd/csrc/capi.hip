@@ -552,10 +552,10 @@ int dbmp_bin_tiles(dbmp_rasterizer* h, const dbmp_prims* prims, const float* ext
   if (tiles <= kDirectMaxLists && (long long)tiles * N <= kDirectMaxTests) {
     // few lists x primitives: build each list directly from the ranges
     if (N <= kDirectStage)
-      hipLaunchKernelGGL(k_tile_build<true>, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX,
+      hipLaunchKernelGGL(k_tile_build<true>, dim3(tiles), dim3(64 * kBuildWaves), 0, st, (const int4*)rg, N, tX,
                          tiles, shift, (long long)capacity, tile_offsets, tile_indices, dtot);
     else
-      hipLaunchKernelGGL(k_tile_build<false>, dim3(tiles), dim3(256), 0, st, (const int4*)rg, N, tX,
+      hipLaunchKernelGGL(k_tile_build<false>, dim3(tiles), dim3(64 * kBuildWaves), 0, st, (const int4*)rg, N, tX,
                          tiles, shift, (long long)capacity, tile_offsets, tile_indices, dtot);
     LAUNCH_CHECK("k_tile_build");
     return 0;
